@@ -1,0 +1,164 @@
+// kernels_sort.hpp — device cell-binned sort with the order of std::stable_sort.
+//
+// HostSideSOA::RemoveDisabledAndSort (simulation/hssoa/host_side_soa.cpp:36-47)
+// removes disabled points and sorts the rest by ProxyPoint::getCellIndex =
+// x*GridY + y (proxypoint2d.cpp:117-123).  Here, per sort:
+//   1. key per point (disabled points go to bucket NC = GridX*GridY) and a slot
+//      inside its cell from an atomic counter,
+//   2. exclusive scans: cell counts -> cs (cell start), disabled flags -> rank,
+//   3. scatter slot indices, then sort each cell's short index run ascending,
+//      which is exactly the stable order (ties keep input order),
+//   4. gather every SoA line into the second buffer in the new order.
+// Disabled points either follow the sorted ones (in input order) or are
+// dropped, as the caller asks.
+#pragma once
+#include "point_ops.hpp"
+
+namespace ice {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wsum, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wid] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int w = 0; w < kScanThreads / 64; w++) {
+        const uint32_t ws = s_wsum[w];
+        if (w < wid) pre += ws;
+        tot += ws;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ in, size_t n,
+                                                              uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t s_w[kScanThreads / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) s += in[base + k];
+    uint32_t tot;
+    block_exclusive_scan(s, s_w, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of bsum[0..nb) in place, bsum[nb] = total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint32_t *__restrict__ bsum, uint32_t nb) {
+    __shared__ uint32_t s_w[kScanThreads / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += kScanTile) {
+        uint32_t v[kScanItems];
+        uint32_t s = 0;
+        for (int k = 0; k < kScanItems; k++) {
+            const uint32_t idx = base + threadIdx.x * kScanItems + k;
+            v[k] = idx < nb ? bsum[idx] : 0u;
+            s += v[k];
+        }
+        uint32_t tot;
+        uint32_t pre = block_exclusive_scan(s, s_w, tot) + carry;
+        for (int k = 0; k < kScanItems; k++) {
+            const uint32_t idx = base + threadIdx.x * kScanItems + k;
+            if (idx < nb) bsum[idx] = pre;
+            pre += v[k];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__restrict__ in, size_t n,
+                                                             const uint32_t *__restrict__ bsum, uint32_t nb,
+                                                             uint32_t *__restrict__ out) {
+    __shared__ uint32_t s_w[kScanThreads / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan(s, s_w, tot) + bsum[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = pre;
+        pre += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+// Points whose cell word lies outside the grid cannot be binned; they join
+// the disabled tail (and raise the left-area error bit).
+__global__ __launch_bounds__(256) void k_sort_keys(Lines L, uint32_t n, int gx, int gy, uint32_t NC,
+                                                   uint32_t *__restrict__ count, uint32_t *__restrict__ key,
+                                                   uint32_t *__restrict__ slot, uint32_t *__restrict__ dflag,
+                                                   Counters *cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = L.u(ICE_IDX_UTILITY, i);
+    const uint32_t c0 = L.u(ICE_IDX_CELL, i);
+    const bool outside = (int)(c0 & 0xffffu) >= gx || (int)(c0 >> 16) >= gy;
+    if (outside && !(u & kDisabled)) atomicOr(&cnt->err, ICE_ERR_POINT_LEFT_AREA);
+    if ((u & kDisabled) || outside) {
+        key[i] = NC;
+        slot[i] = 0;
+        dflag[i] = 1;
+    } else {
+        const uint32_t c = L.u(ICE_IDX_CELL, i);
+        const uint32_t k = (c & 0xffffu) * (uint32_t)gy + (c >> 16);
+        key[i] = k;
+        slot[i] = atomicAdd(&count[k], 1u);
+        dflag[i] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, uint32_t NC, const uint32_t *__restrict__ key,
+                                                      const uint32_t *__restrict__ slot,
+                                                      const uint32_t *__restrict__ cs,
+                                                      const uint32_t *__restrict__ drank,
+                                                      uint32_t *__restrict__ order) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = key[i];
+    const uint32_t pos = (k == NC) ? cs[NC] + drank[i] : cs[k] + slot[i];
+    order[pos] = i;
+}
+
+// Ascending input index inside each cell = the stable order.
+__global__ __launch_bounds__(256) void k_sort_cells(uint32_t NC, const uint32_t *__restrict__ cs,
+                                                    uint32_t *__restrict__ order) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= NC) return;
+    const uint32_t b = cs[k], e = cs[k + 1];
+    for (uint32_t p = b + 1; p < e; p++) {
+        const uint32_t v = order[p];
+        uint32_t q = p;
+        while (q > b && order[q - 1] > v) {
+            order[q] = order[q - 1];
+            q--;
+        }
+        order[q] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sort_gather(Lines src, Lines dst, int nlines, uint32_t n_out,
+                                                     const uint32_t *__restrict__ order, uint32_t *__restrict__ perm) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_out) return;
+    const uint32_t s = order[p];
+    dst.set_u(ICE_IDX_UTILITY, p, src.u(ICE_IDX_UTILITY, s) & ~kInternalBits);
+    for (int l = 1; l < nlines; l++) dst(l, p) = src(l, s);
+    if (perm) perm[p] = s;
+}
+
+}  // namespace ice

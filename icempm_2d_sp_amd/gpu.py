@@ -1,0 +1,382 @@
+"""Host-side mirror of the reference's GPU_Implementation5 / GPU_Partition /
+HostSideSOA call surface (simulation/gpu_implementation5.h:23-68,
+gpu_partition.h:63-134, hssoa/host_side_soa.h:51-74) over the C-ABI.
+
+Same method names, argument meaning and error behaviour (failures raise, as
+the reference throws std::runtime_error).  One partition on one GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .params import SimParams
+
+
+class HostSideSOA:
+    """Host SoA [nPtsArrays][capacity] of float32 (host_side_soa.h:51-74)."""
+
+    def __init__(self, nlines: int = abi.N_LINES):
+        self.nlines = nlines
+        self.host_buffer = np.zeros((nlines, 0), np.float32)
+        self.capacity = 0
+        self.size = 0
+
+    def Allocate(self, pts_capacity: int):
+        self.capacity = pts_capacity
+        self.size = 0
+        self.host_buffer = np.zeros((self.nlines, pts_capacity), np.float32)
+
+    def getPointerToLine(self, idx_line: int) -> np.ndarray:
+        return self.host_buffer[idx_line]
+
+    def line_u32(self, idx_line: int) -> np.ndarray:
+        return self.host_buffer[idx_line].view(np.uint32)
+
+    def RemoveDisabledAndSort(self, GridY: int):
+        """host_side_soa.cpp:36-47, with a stable order."""
+        n = self.size
+        util = self.line_u32(abi.IDX_UTILITY)[:n]
+        keep = np.nonzero((util & abi.STATUS_DISABLED) == 0)[0]
+        cell = self.line_u32(abi.IDX_CELL)[keep]
+        key = (cell & 0xFFFF).astype(np.int64) * GridY + (cell >> 16)
+        order = keep[np.argsort(key, kind="stable")]
+        self.host_buffer[:, : order.size] = self.host_buffer[:, order]
+        self.size = order.size
+
+    def convertToIntegerCellFormat(self, h: float):
+        """proxypoint2d.cpp:125-139 for every point: world position -> (cell, offset)."""
+        n = self.size
+        x = self.host_buffer[abi.IDX_POSX, :n].astype(np.float32)
+        y = self.host_buffer[abi.IDX_POSX + 1, :n].astype(np.float32)
+        hinv = np.float32(1.0) / np.float32(h)
+        xi = (x * hinv + np.float32(0.5)).astype(np.uint32)
+        yi = (y * hinv + np.float32(0.5)).astype(np.uint32)
+        self.line_u32(abi.IDX_CELL)[:n] = (yi << 16) | xi
+        self.host_buffer[abi.IDX_POSX, :n] = x * hinv - xi.astype(np.float32)
+        self.host_buffer[abi.IDX_POSX + 1, :n] = y * hinv - yi.astype(np.float32)
+
+
+class GPU_Partition:
+    """One partition = one IceHandle (gpu_partition.h:63-134)."""
+
+    prms: SimParams | None = None
+
+    def __init__(self):
+        self.lib = abi.load()
+        self.h = C.c_void_p()
+        self.PartitionID = 0
+        self.Device = 0
+        self.nPts_partition = 0
+        self.GridX_partition = 0
+        self.GridX_offset = 0
+        self.error_code = 0
+        self.disabled_points_count = 0
+        self.nPtsPitch = 0
+        self.nGridPitch = 0
+        self.timing_10_P2GAndHalo = 0.0
+        self.timing_20_acceptHalo = 0.0
+        self.timing_30_updateGrid = 0.0
+        self.timing_40_G2P = 0.0
+        self.timing_60_ptsSent = 0.0
+        self.timing_70_ptsAccepted = 0.0
+        self.timing_stepTotal = 0.0
+        self._initialized = False
+
+    def _chk(self, rc, what):
+        abi.check(rc, self.h, what)
+
+    def initialize(self, device: int, partition: int):
+        if self._initialized:
+            raise RuntimeError("GPU_Partition double initialization")
+        self.PartitionID, self.Device = partition, device
+        cp = self.prms.to_c()
+        rc = self.lib.ice_create(C.byref(cp), device, C.byref(self.h))
+        abi.check(rc, None, "GPU_Partition initialization failure")
+        self.lib.ice_enable_timings(self.h, 1)
+        self._initialized = True
+
+    def destroy(self):
+        if self.h:
+            self.lib.ice_destroy(self.h)
+            self.h = C.c_void_p()
+            self._initialized = False
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def allocate(self, n_points_capacity: int, grid_x_capacity: int):
+        self._chk(self.lib.ice_allocate(self.h, n_points_capacity), "GPU_Partition allocate")
+        info = self.info()
+        self.nPtsPitch, self.nGridPitch = int(info.point_pitch), int(info.grid_pitch)
+
+    def info(self) -> abi.IceInfo:
+        i = abi.IceInfo()
+        self._chk(self.lib.ice_get_info(self.h, C.byref(i)), "info")
+        return i
+
+    def transfer_points_from_soa_to_device(self, hssoa: HostSideSOA, point_idx_offset: int):
+        buf = hssoa.host_buffer
+        view = np.ascontiguousarray(buf[:, point_idx_offset: point_idx_offset + self.nPts_partition])
+        self._chk(self.lib.ice_upload_points(self.h, abi.ptr(view), view.shape[1], self.nPts_partition,
+                                             view.shape[0]), "transfer_points_from_soa_to_device")
+        self.disabled_points_count = 0
+
+    def transfer_grid_data_to_device(self, gpu: "GPU_Implementation5"):
+        st = np.ascontiguousarray(gpu.grid_status_buffer, np.uint8)
+        nrm = np.ascontiguousarray(gpu.grid_boundary_normals, np.float32) if gpu.grid_boundary_normals.size else None
+        self._chk(self.lib.ice_upload_grid(self.h, abi.ptr(st), abi.ptr(nrm)), "transfer_grid_data_to_device")
+
+    def update_constants(self):
+        cp = self.prms.to_c()
+        self._chk(self.lib.ice_update_constants(self.h, C.byref(cp)), "cuda_update_constants")
+        self.error_code = 0
+
+    def update_current_field(self, current: np.ndarray | None, wind_stress: np.ndarray | None = None):
+        cur = None if current is None else np.ascontiguousarray(current, np.float32)
+        wnd = None if wind_stress is None else np.ascontiguousarray(wind_stress, np.float32)
+        self._chk(self.lib.ice_upload_forcing(self.h, abi.ptr(cur), abi.ptr(wnd)), "update_current_field")
+
+    def transfer_from_device(self, hssoa: HostSideSOA, point_idx_offset: int, boundary_forces: np.ndarray):
+        n = self.info().n_points
+        if point_idx_offset + n > hssoa.capacity:
+            raise RuntimeError("transfer_from_device() HSSOA capacity")
+        out = np.zeros((hssoa.nlines, max(n, 1)), np.float32)
+        err = C.c_uint32(0)
+        dis = C.c_int32(0)
+        self._chk(self.lib.ice_download(self.h, abi.ptr(out), out.shape[1], hssoa.nlines, abi.ptr(boundary_forces),
+                                        C.byref(err), C.byref(dis)), "transfer_from_device")
+        hssoa.host_buffer[:, point_idx_offset: point_idx_offset + n] = out[:, :n]
+        self.nPts_partition = n
+        self.error_code = err.value
+        self.disabled_points_count = dis.value
+
+    def reset_grid(self):
+        self._chk(self.lib.ice_reset_grid(self.h), "cuda_reset_grid error")
+
+    def clear_force_accumulator(self):
+        self._chk(self.lib.ice_clear_force_accumulator(self.h), "clear_force_accumulator()")
+
+    def p2g(self):
+        self._chk(self.lib.ice_p2g(self.h), "p2g kernel")
+
+    def update_nodes(self, simulation_time: float, windSpeed: float = 0.0, windAngle: float = 0.0):
+        self._chk(self.lib.ice_update_nodes(self.h, simulation_time, windSpeed, windAngle), "update_nodes")
+
+    def g2p(self, recordPQ: bool, enablePointTransfer: bool = False, applyGlensLaw: int = 0):
+        self._chk(self.lib.ice_g2p(self.h, int(recordPQ), int(enablePointTransfer), int(applyGlensLaw)),
+                  "g2p kernel")
+
+    def reset_timings(self):
+        self._chk(self.lib.ice_reset_timings(self.h), "reset_timings")
+        for k in ("timing_10_P2GAndHalo", "timing_20_acceptHalo", "timing_30_updateGrid", "timing_40_G2P",
+                  "timing_60_ptsSent", "timing_70_ptsAccepted", "timing_stepTotal"):
+            setattr(self, k, 0.0)
+
+    def record_timings(self, enablePointTransfer: bool = False):
+        out = (C.c_double * 8)()
+        self._chk(self.lib.ice_get_timings(self.h, out), "record_timings")
+        self.timing_10_P2GAndHalo = out[1]
+        self.timing_30_updateGrid = out[2]
+        self.timing_40_G2P = out[3]
+        self.timing_stepTotal = out[4]
+
+    def normalize_timings(self, cycles: int):
+        coeff = 1000.0 / cycles
+        for k in ("timing_10_P2GAndHalo", "timing_20_acceptHalo", "timing_30_updateGrid", "timing_40_G2P",
+                  "timing_60_ptsSent", "timing_70_ptsAccepted", "timing_stepTotal"):
+            setattr(self, k, getattr(self, k) * coeff)
+
+
+class GPU_Implementation5:
+    """gpu_implementation5.h:23-68 with one partition."""
+
+    def __init__(self, prms: SimParams, nlines: int | None = None):
+        self.prms = prms
+        GPU_Partition.prms = prms
+        self.partitions: list[GPU_Partition] = []
+        nl = nlines or (abi.N_LINES_MAX if prms.flags & abi.FLAG_COHESION_LINE else abi.N_LINES)
+        self.hssoa = HostSideSOA(nl)
+        self.error_code = 0
+        N = prms.GridXTotal * prms.GridYTotal
+        self.grid_status_buffer = np.full(N, abi.GRID_STATUS_WATER, np.uint8)
+        self.grid_boundary_normals = np.zeros(0, np.float32)
+        self.grid_boundary_forces = np.zeros(2 * N, np.float32)
+        self.current = None
+        self.wind_stress = None
+        self.transfer_completion_callback = None
+
+    def allocate_host_arrays_grid(self):
+        N = self.prms.GridXTotal * self.prms.GridYTotal
+        self.grid_status_buffer = np.full(N, abi.GRID_STATUS_WATER, np.uint8)
+        self.grid_boundary_normals = np.zeros(2 * N, np.float32)
+        self.grid_boundary_forces = np.zeros(2 * N, np.float32)
+
+    def allocate_host_arrays_points(self, nPtsInitial: int):
+        self.hssoa.Allocate(nPtsInitial)
+
+    def initialize(self, device: int = 0):
+        self.partitions = [GPU_Partition()]
+        self.partitions[0].initialize(device, 0)
+
+    def split_hssoa_into_partitions(self):
+        p = self.partitions[0]
+        p.nPts_partition = self.hssoa.size
+        p.GridX_partition = self.prms.GridXTotal
+        p.disabled_points_count = 0
+
+    def allocate_device_arrays(self):
+        p = self.partitions[0]
+        p.allocate(self.hssoa.capacity, p.GridX_partition)
+
+    def transfer_to_device(self):
+        p = self.partitions[0]
+        p.transfer_points_from_soa_to_device(self.hssoa, 0)
+        p.transfer_grid_data_to_device(self)
+
+    def transfer_wind_and_current_data_to_device(self):
+        for p in self.partitions:
+            p.update_current_field(self.current, self.wind_stress)
+
+    def transfer_from_device(self, elapsed_cycles: int):
+        offset = 0
+        for p in self.partitions:
+            p.transfer_from_device(self.hssoa, offset, self.grid_boundary_forces)
+            offset += p.nPts_partition
+            if p.error_code:
+                self.error_code = p.error_code
+        self.hssoa.size = offset
+        # boundary-force normalisation (gpu_implementation5.cpp:219-221)
+        self.grid_boundary_forces /= (elapsed_cycles * self.prms.InitialTimeStep)
+        if self.transfer_completion_callback:
+            self.transfer_completion_callback()
+
+    def synchronize(self):
+        for p in self.partitions:
+            p._chk(p.lib.ice_synchronize(p.h), "synchronize")
+
+    def update_constants(self):
+        self.error_code = 0
+        for p in self.partitions:
+            p.update_constants()
+
+    def reset_grid(self):
+        for p in self.partitions:
+            p.reset_grid()
+
+    def reset_timings(self):
+        for p in self.partitions:
+            p.reset_timings()
+
+    def clear_force_accumulator(self):
+        for p in self.partitions:
+            p.clear_force_accumulator()
+
+    def p2g(self):
+        self.partitions[0].p2g()
+
+    def update_nodes(self, simulation_time: float, windSpeed: float = 0.0, windAngle: float = 0.0):
+        for p in self.partitions:
+            p.update_nodes(simulation_time, windSpeed, windAngle)
+
+    def g2p(self, recordPQ: bool, enablePointTransfer: bool = False, applyGlensLaw: int = 0):
+        for p in self.partitions:
+            p.g2p(recordPQ, enablePointTransfer, applyGlensLaw)
+
+    def record_timings(self, enablePointTransfer: bool = False):
+        for p in self.partitions:
+            p.record_timings(enablePointTransfer)
+
+
+class Simulation:
+    """A scene on one GPU through the C-ABI: upload, run, download."""
+
+    def __init__(self, scene, device: int = 0, *, flags: int | None = None, tile=None, capacity: int | None = None):
+        self.lib = abi.load()
+        self.scene = scene
+        prm = scene.params if flags is None else scene.params.replace(flags=flags)
+        self.params = prm
+        self.cp = prm.to_c()
+        self.h = C.c_void_p()
+        abi.check(self.lib.ice_create(C.byref(self.cp), device, C.byref(self.h)), None, "ice_create")
+        if tile is not None:
+            self._chk(self.lib.ice_set_tile(self.h, tile[0], tile[1]), "ice_set_tile")
+        self.nlines = scene.nlines
+        self._chk(self.lib.ice_allocate(self.h, capacity or scene.n), "ice_allocate")
+        self._chk(self.lib.ice_upload_grid(self.h, abi.ptr(np.ascontiguousarray(scene.status)), None),
+                  "ice_upload_grid")
+        cur = None if scene.current is None else np.ascontiguousarray(scene.current, np.float32)
+        wnd = None if scene.wind is None else np.ascontiguousarray(scene.wind, np.float32)
+        self._chk(self.lib.ice_upload_forcing(self.h, abi.ptr(cur), abi.ptr(wnd)), "ice_upload_forcing")
+        soa = np.ascontiguousarray(scene.soa, np.float32)
+        self._chk(self.lib.ice_upload_points(self.h, abi.ptr(soa), soa.shape[1], soa.shape[1], soa.shape[0]),
+                  "ice_upload_points")
+
+    def _chk(self, rc, what):
+        abi.check(rc, self.h, what)
+
+    def close(self):
+        if self.h:
+            self.lib.ice_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def substep(self, t: float, record_pq: bool = False):
+        self._chk(self.lib.ice_substep(self.h, t, int(record_pq)), "ice_substep")
+
+    def run(self, first_step: int, nsteps: int, graph: bool = True):
+        self._chk(self.lib.ice_run(self.h, first_step, nsteps, int(graph)), "ice_run")
+
+    def sync(self):
+        self._chk(self.lib.ice_synchronize(self.h), "ice_synchronize")
+
+    def info(self) -> abi.IceInfo:
+        i = abi.IceInfo()
+        self._chk(self.lib.ice_get_info(self.h, C.byref(i)), "ice_get_info")
+        return i
+
+    def download(self):
+        """(points [nlines, n], forces [2, N], error word, disabled count)."""
+        n = self.info().n_points
+        out = np.zeros((self.nlines, max(n, 1)), np.float32)
+        N = self.params.GridXTotal * self.params.GridYTotal
+        forces = np.zeros((2, N), np.float32)
+        err, dis = C.c_uint32(0), C.c_int32(0)
+        self._chk(self.lib.ice_download(self.h, abi.ptr(out), out.shape[1], self.nlines, abi.ptr(forces),
+                                        C.byref(err), C.byref(dis)), "ice_download")
+        return out[:, :n], forces, err.value, dis.value
+
+    def grid(self):
+        N = self.params.GridXTotal * self.params.GridYTotal
+        mpp = np.zeros((3, N), np.float32)
+        vel = np.zeros((2, N), np.float32)
+        self._chk(self.lib.ice_download_grid(self.h, abi.ptr(mpp), abi.ptr(vel)), "ice_download_grid")
+        return mpp, vel
+
+    def sort(self, want_perm: bool = False):
+        n0 = self.info().n_points
+        perm = np.zeros(max(n0, 1), np.uint32) if want_perm else None
+        nout = C.c_uint32(0)
+        self._chk(self.lib.ice_sort(self.h, C.byref(nout), abi.ptr(perm)), "ice_sort")
+        return perm[: nout.value] if want_perm else nout.value
+
+    def timings(self):
+        out = (C.c_double * 8)()
+        self._chk(self.lib.ice_get_timings(self.h, out), "ice_get_timings")
+        return list(out)
+
+    def enable_timings(self, on: bool = True):
+        self._chk(self.lib.ice_enable_timings(self.h, int(on)), "ice_enable_timings")
+
+    def reset_timings(self):
+        self._chk(self.lib.ice_reset_timings(self.h), "ice_reset_timings")

@@ -1,0 +1,205 @@
+/*
+ * icempm.h — C-ABI of the MI355X-native single-partition MPM substep.
+ *
+ * Plain C: pointers, sizes and ints only, no C++ or torch types.  Every entry
+ * point returns 0 on success and a negative ICE_E_* code on failure; it never
+ * throws across the ABI.  ice_last_error(h) gives the message of the last
+ * failure on a handle (ice_last_error(NULL) the last failure of ice_create).
+ * A handle is not thread-safe: drive it from one host thread.  Downloads are
+ * synchronous when they return.
+ *
+ * Each declaration names the reference interface it replaces
+ * (igorg520b/iceMPM_2d_sp, file:line).
+ */
+#ifndef ICEMPM_H
+#define ICEMPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- particle SoA layout (hssoa), one line of `pitch` 4-byte slots per index.
+ * Same indices as SimParams (simulation/parameters_sim.h:65-76) with
+ * t_PointReal = float (parameters_sim.h:26).  Integer lines (utility, cell,
+ * point index) are uint32 bit patterns stored in the float slots. */
+#define ICE_IDX_UTILITY    0   /* bit16 crushed, bit17 disabled, bit18 error/weakened */
+#define ICE_IDX_CELL       1   /* (y << 16) | x of the nearest node */
+#define ICE_IDX_POINT      2   /* original point index */
+#define ICE_IDX_P          3
+#define ICE_IDX_Q          4
+#define ICE_IDX_JP_INV     5
+#define ICE_IDX_POSX       6   /* cell-local position in [-0.5, 0.5], 6..7 */
+#define ICE_IDX_VELX       8   /* 8..9 */
+#define ICE_IDX_FE00       10  /* Fe row-major, 10..13 */
+#define ICE_IDX_BP00       14  /* Bp (APIC C) row-major, 14..17 */
+#define ICE_IDX_THICKNESS  18
+#define ICE_N_LINES        19  /* nPtsArrays of the reference */
+#define ICE_IDX_COHESION   19  /* optional per-particle cohesion line (Pa), build addition */
+#define ICE_N_LINES_MAX    20
+
+#define ICE_STATUS_CRUSHED  0x10000u  /* kernels.cuh:11 */
+#define ICE_STATUS_DISABLED 0x20000u  /* kernels.cuh:12 */
+#define ICE_STATUS_ERROR    0x40000u  /* kernels.cuh:13 */
+
+/* error bits of the device error word (kernels.cuh:17-26) */
+#define ICE_ERR_POINT_POS_NAN     0x0001u
+#define ICE_ERR_POINT_VEL_NAN     0x0002u
+#define ICE_ERR_POINT_JUMP_CELLS  0x0004u
+#define ICE_ERR_POINT_LEFT_AREA   0x0008u
+#define ICE_ERR_POINT_BP_NAN      0x0010u
+#define ICE_ERR_POINT_FE_NAN      0x0020u
+#define ICE_ERR_GRID_P2G_NAN_VEL  0x0100u
+#define ICE_ERR_GRID_P2G_NAN_MASS 0x0200u
+#define ICE_ERR_GRID_NAN          0x0400u
+
+/* grid status byte: 100 = modelled water, anything else = sticky wall whose
+ * value is the boundary region id (snapshotmanager.cpp:129-144). */
+#define ICE_GRID_STATUS_WATER 100
+
+/* ---- flags in IceParams.flags */
+#define ICE_FLAG_COHESION_LINE  0x1u  /* line 19 holds per-particle cohesion */
+#define ICE_FLAG_DETERMINISTIC  0x2u  /* fixed-point P2G accumulation (bit-reproducible) */
+#define ICE_FLAG_NAIVE          0x4u  /* unsorted atomic P2G + separate update + G2P kernels */
+
+/* return codes */
+#define ICE_OK          0
+#define ICE_E_ARG      -1
+#define ICE_E_HIP      -2
+#define ICE_E_STATE    -3
+#define ICE_E_CAPACITY -4
+#define ICE_E_DEVICE   -5
+
+/* Model constants: the fields of SimParams (parameters_sim.h:78-117) that the
+ * substep reads, with the FP64 types of the reference.  The block marked
+ * "derived" is filled by ice_params_derive from the others. */
+typedef struct IceParams {
+    int32_t GridXTotal, GridYTotal;            /* grid nodes along x and y */
+    double  cellsize;                          /* h, m */
+    double  InitialTimeStep;                   /* dt, s */
+    double  AnimationFramePeriod;              /* s; sets UpdateEveryNthStep */
+    double  IceDensity, PoissonsRatio, YoungsModulus;
+    double  IceCompressiveStrength, IceTensileStrength, IceShearStrength, IceTensileStrength2;
+    double  DP_phi, DP_threshold_p;            /* degrees; Pa */
+    double  RidgeFormationCoeff;
+    double  ParticleVolume;                    /* m^2 per particle */
+    double  sea_water_density;
+    double  waterDragEffectiveLinear;          /* 1/s */
+    double  waterDragEffectiveQuadratic;       /* 1/m */
+    double  windStressToAccel;                 /* 1/(rho_ice*h_ice): wind stress (Pa) -> m/s^2; build addition */
+    /* derived (parameters_sim.cpp:122-140) */
+    double  dt_vol_Dpinv, vmax, lambda, mu, kappa, ParticleMass, cellsize_inv, Dp_inv;
+    int32_t UpdateEveryNthStep;
+    /* launch shape and mode */
+    int32_t tpb_P2G, tpb_Upd, tpb_G2P;         /* parameters_sim.cpp:46-48; naive path only */
+    uint32_t flags;                            /* ICE_FLAG_* */
+} IceParams;
+
+typedef struct IceHandle IceHandle;
+
+/* Fill defaults of SimParams::Reset (parameters_sim.cpp:7-55) into *p. */
+int ice_params_reset(IceParams *p);
+/* SimParams::ComputeHelperVariables + ComputeLame (parameters_sim.cpp:122-140). */
+int ice_params_derive(IceParams *p);
+
+/* GPU_Implementation5::initialize + GPU_Partition::initialize
+ * (gpu_implementation5.cpp:94-128; gpu_partition.cpp:234-256) and
+ * update_constants (gpu_partition.cpp:140-170): one partition on `device`. */
+int ice_create(const IceParams *p, int device, IceHandle **out);
+/* GPU_Partition destructor (gpu_partition.cpp:214-232). */
+int ice_destroy(IceHandle *h);
+const char *ice_last_error(const IceHandle *h);
+
+/* GPU_Partition::allocate (gpu_partition.cpp:259-293).  Device pitch is
+ * reported by ice_get_info. */
+int ice_allocate(IceHandle *h, uint32_t n_points_capacity);
+
+/* GPU_Partition::transfer_points_from_soa_to_device (gpu_partition.cpp:81-106):
+ * `n` points from a host SoA with `host_pitch` slots per line and `n_lines`
+ * lines (19, or 20 with the cohesion line).  Sets the disabled count to 0, as
+ * split_hssoa_into_partitions does (gpu_implementation5.cpp:137).  Unless
+ * ICE_FLAG_NAIVE is set, the points are then put in cell order on the device;
+ * ICE_IDX_POINT identifies them in later downloads. */
+int ice_upload_points(IceHandle *h, const float *soa, size_t host_pitch, uint32_t n, int n_lines);
+/* GPU_Partition::transfer_grid_data_to_device (gpu_partition.cpp:108-137):
+ * status [gx*gy] bytes, node index x*gy+y; normals [2][gx*gy] or NULL. */
+int ice_upload_grid(IceHandle *h, const uint8_t *status, const float *normals);
+/* GPU_Partition::update_current_field (gpu_partition.cpp:174-196) plus the
+ * wind-stress field the reference declares but never applies
+ * (gpu_implementation5.cpp:51-71): each [2][gx*gy] (x then y) or NULL = zero. */
+int ice_upload_forcing(IceHandle *h, const float *current, const float *wind_stress);
+/* GPU_Partition::update_constants (gpu_partition.cpp:140-170): new constants,
+ * same grid size. */
+int ice_update_constants(IceHandle *h, const IceParams *p);
+
+/* ---- per-substep phases (gpu_implementation5.cpp:18-81) */
+int ice_reset_grid(IceHandle *h);                                   /* gpu_partition.cpp:300-308 */
+int ice_p2g(IceHandle *h);                                          /* gpu_partition.cpp:322-334 */
+int ice_update_nodes(IceHandle *h, float simulation_time, float windSpeed, float windAngle); /* :336-348 */
+int ice_g2p(IceHandle *h, int recordPQ, int enablePointTransfer, int applyGlensLaw);        /* :350-365 */
+/* One whole substep (reset_grid, p2g, update_nodes, g2p) as one call; the
+ * sequence Model::Step runs (model.cpp:18-39). */
+int ice_substep(IceHandle *h, double simulation_time, int recordPQ);
+/* `nsteps` substeps starting at step index `first_step` (t = dt*step),
+ * recordPQ on steps that are multiples of UpdateEveryNthStep; replays a
+ * captured HIP graph when use_graph != 0. */
+int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph);
+
+/* ---- frame boundary */
+int ice_clear_force_accumulator(IceHandle *h);                      /* gpu_partition.cpp:310-319 */
+/* GPU_Partition::transfer_from_device (gpu_partition.cpp:17-59): points into a
+ * host SoA (NULL to skip), boundary forces [2][gx*gy] (NULL to skip), the
+ * error word and the disabled count.  Synchronous. */
+int ice_download(IceHandle *h, float *soa, size_t host_pitch, int n_lines,
+                 float *boundary_forces, uint32_t *error_code, int32_t *disabled_count);
+/* Grid fields after the last substep: [3][gx*gy] m, px, py as P2G left them
+ * (momentum), or NULL; [2][gx*gy] node velocities after the update, or NULL. */
+int ice_download_grid(IceHandle *h, float *mpp, float *velocity);
+int ice_synchronize(IceHandle *h);                                  /* gpu_implementation5.cpp:227-234 */
+
+/* Cell-binned sort on the device: drop disabled points and stable-sort the
+ * rest by cell index x*GridY+y (HostSideSOA::RemoveDisabledAndSort,
+ * host_side_soa.cpp:36-47, with std::stable_sort order).  *n_out = points
+ * kept.  perm (NULL to skip) receives, per output slot, the input slot. */
+int ice_sort(IceHandle *h, uint32_t *n_out, uint32_t *perm);
+
+/* ---- timings (GPU_Partition::reset_timings / record_timings /
+ * normalize_timings, gpu_partition.cpp:370-459), HIP events on the handle's
+ * stream around each phase, accumulated since the last reset:
+ * out[0] ms in the main P2G kernel, out[1] ms in the P2G phase (grid clear,
+ * P2G, off-tile P2G), out[2] ms in the node update (naive) or off-tile G2P,
+ * out[3] ms in the main G2P kernel, out[4] ms of whole substeps, out[5]
+ * substeps recorded, out[6..7] 0.  Graph replays record nothing. */
+int ice_reset_timings(IceHandle *h);
+int ice_enable_timings(IceHandle *h, int enable);
+int ice_get_timings(IceHandle *h, double out[8]);
+
+/* Tile of grid nodes per workgroup of the sorted path (width along x, height
+ * along y).  Supported: 8x8, 8x16, 16x8, 16x16, 16x32, 32x16, 32x32, 32x64,
+ * 64x32, 64x64.  No reference counterpart (the reference's only launch knobs
+ * are tpb_P2G/Upd/G2P, parameters_sim.cpp:111-113). */
+int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
+
+/* ---- introspection */
+typedef struct IceInfo {
+    uint32_t n_points;           /* nPts_partition */
+    uint32_t capacity;           /* points the device buffer holds */
+    uint64_t point_pitch;        /* device slots per line (nPtsPitch) */
+    uint64_t grid_pitch;         /* device slots per grid line (nGridPitch) */
+    int32_t  grid_x, grid_y;
+    int32_t  device;
+    int32_t  sorted;             /* 1 when points are in cell order */
+    int32_t  steps_since_sort;
+    int32_t  n_sorts;            /* device sorts done since create */
+    uint32_t flags;
+    int32_t  tile_w, tile_h;     /* tile of the sorted path */
+    uint32_t n_offtile;          /* points handled off-tile until the next sort */
+} IceInfo;
+int ice_get_info(IceHandle *h, IceInfo *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICEMPM_H */

@@ -1,0 +1,69 @@
+"""Point/grid comparison helpers for the parity tests.
+
+Points are matched by their ICE_IDX_POINT line (the device may reorder them by
+cell).  Float fields are compared as |a - b| <= atol + rtol * scale, where
+scale is the field's largest magnitude in the reference (so nearly-zero entries
+are judged against the field's size, not their own).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from icempm_2d_sp_amd import abi
+
+# (name, lines, rtol relative to the field scale) for FP32 vs the FP64 oracle
+FIELDS_1STEP = [
+    ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 2e-6),
+    ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 2e-5),
+    ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 2e-6),
+    ("Bp", list(range(abi.IDX_BP00, abi.IDX_BP00 + 4)), 1e-4),
+    ("Jp_inv", [abi.IDX_JP_INV], 2e-5),
+    ("P", [abi.IDX_P], 1e-3),
+    ("Q", [abi.IDX_Q], 1e-3),
+]
+
+
+def by_point(soa: np.ndarray) -> np.ndarray:
+    idx = soa[abi.IDX_POINT].view(np.uint32)
+    order = np.argsort(idx, kind="stable")
+    return soa[:, order]
+
+
+def u32(soa: np.ndarray, line: int) -> np.ndarray:
+    return np.ascontiguousarray(soa[line]).view(np.uint32)
+
+
+def field_errors(gpu: np.ndarray, ref: np.ndarray, lines) -> tuple[np.ndarray, float]:
+    """Per-point max error over `lines` divided by the field scale."""
+    g = gpu[lines].astype(np.float64)
+    r = ref[lines].astype(np.float64)
+    scale = max(np.abs(r).max(), 1e-30)
+    return np.abs(g - r).max(axis=0) / scale, scale
+
+
+def compare_points(gpu: np.ndarray, ref: np.ndarray, fields=FIELDS_1STEP, *, max_bad_frac: float = 0.0,
+                   max_flag_mismatch: float = 0.0, report: dict | None = None) -> dict:
+    """Assert FP32 device points match the reference points.
+
+    max_bad_frac: fraction of points allowed outside a field's tolerance
+    (points whose return-map branch flipped on rounding); 0 for one substep.
+    """
+    gpu = by_point(gpu)
+    ref = by_point(ref)
+    assert gpu.shape[1] == ref.shape[1], f"point count {gpu.shape[1]} != {ref.shape[1]}"
+    np.testing.assert_array_equal(u32(gpu, abi.IDX_POINT), u32(ref, abi.IDX_POINT))
+    out = {} if report is None else report
+    ug, ur = u32(gpu, abi.IDX_UTILITY), u32(ref, abi.IDX_UTILITY)
+    flag_mismatch = float(np.mean(ug != ur))
+    out["flag_mismatch"] = flag_mismatch
+    assert flag_mismatch <= max_flag_mismatch, f"utility words differ on {flag_mismatch:.2e} of points"
+    cell_mismatch = float(np.mean(u32(gpu, abi.IDX_CELL) != u32(ref, abi.IDX_CELL)))
+    out["cell_mismatch"] = cell_mismatch
+    assert cell_mismatch <= max_bad_frac, f"cells differ on {cell_mismatch:.2e} of points"
+    for name, lines, rtol in fields:
+        err, scale = field_errors(gpu, ref, lines)
+        bad = float(np.mean(err > rtol))
+        out[name] = {"max": float(err.max()), "median": float(np.median(err)), "bad": bad, "scale": scale}
+        assert bad <= max_bad_frac, (f"{name}: {bad:.2e} of points beyond rtol {rtol} "
+                                     f"(max {err.max():.3e}, scale {scale:.3e})")
+    return out
