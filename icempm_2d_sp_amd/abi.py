@@ -93,6 +93,7 @@ SIGNATURES = {
     "ice_enable_timings": (C.c_int, [_P, C.c_int]),
     "ice_get_timings": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "ice_set_tile": (C.c_int, [_P, C.c_int, C.c_int]),
+    "ice_set_option": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "ice_get_info": (C.c_int, [_P, C.POINTER(IceInfo)]),
 }
 

@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libicempm.so")
 
 SOURCES = ["icempm_abi.hip"]
 HEADERS = ["device_math.hpp", "point_ops.hpp", "kernels_naive.hpp", "kernels_tiled.hpp",
-           "kernels_sort.hpp"]
+           "kernels_sort.hpp", "kernels_p2g_cells.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

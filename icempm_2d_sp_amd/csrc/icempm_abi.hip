@@ -18,6 +18,7 @@
 #include "kernels_naive.hpp"
 #include "kernels_tiled.hpp"
 #include "kernels_sort.hpp"
+#include "kernels_p2g_cells.hpp"
 
 using namespace ice;
 
@@ -64,6 +65,7 @@ struct IceHandle {
     int steps_since_sort = 0, n_sorts = 0;
     // tiles
     int tile_w = 32, tile_h = 32;
+    int p2g_point_lanes = 0;       // 1: one lane per point in the tiled P2G (the default is one per cell)
     FixScale fix{};
     // phase state
     double pending_t = 0.0;
@@ -200,16 +202,21 @@ int launch_fill_u32(IceHandle *h, uint32_t *a, size_t n, uint32_t v) {
 // ---- tile dispatch ----------------------------------------------------------
 
 using P2GFn = void (*)(DevParams, Lines, const uint32_t *, float *, float *, float *, Counters *, int, int, FixScale);
+using P2GCellFn = void (*)(DevParams, Lines, const uint32_t *, float *, float *, float *, Counters *, int, int,
+                           FixScale, const uint32_t *);
 using G2PFn = void (*)(DevParams, StepArg, Lines, const uint32_t *, GridIn, float *, float *, Counters *, uint32_t *,
                        int, int);
 
 struct TileKernels {
     int w, h;
-    P2GFn p2g, p2g_det;
+    P2GFn p2g, p2g_det;            // one lane per point
+    P2GCellFn p2gc, p2gc_det;      // one lane per cell
     G2PFn g2p;
 };
 
-#define TK(W, H) {W, H, k_p2g_tiled<W, H, false>, k_p2g_tiled<W, H, true>, k_g2p_tiled<W, H>}
+#define TK(W, H)                                                                                      \
+    {W, H, k_p2g_tiled<W, H, false>, k_p2g_tiled<W, H, true>, k_p2g_cells<W, H, false>,             \
+     k_p2g_cells<W, H, true>, k_g2p_tiled<W, H>}
 const TileKernels kTiles[] = {TK(8, 8),   TK(8, 16),  TK(16, 8),  TK(16, 16), TK(16, 32),
                               TK(32, 16), TK(32, 32), TK(32, 64), TK(64, 32), TK(64, 64)};
 #undef TK
@@ -337,14 +344,22 @@ int phase_p2g(IceHandle *h) {
     }
     const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
     const int ntx = (P.gx + h->tile_w - 1) / h->tile_w, nty = (P.gy + h->tile_h - 1) / h->tile_h;
-    P2GFn fn = h->det() ? tk->p2g_det : tk->p2g;
-    hipLaunchKernelGGL(fn, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, h->lines(), h->s_cs, h->gline(G_M),
-                       h->gline(G_PX), h->gline(G_PY), h->cnt, ntx, nty, h->fix);
+    if (h->p2g_point_lanes && !h->det()) {
+        hipLaunchKernelGGL(tk->p2g, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, h->lines(), h->s_cs,
+                           h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt, ntx, nty, h->fix);
+    } else {
+        P2GCellFn fn = h->det() ? tk->p2gc_det : tk->p2gc;
+        hipLaunchKernelGGL(fn, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, h->lines(), h->s_cs,
+                           h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt, ntx, nty, h->fix,
+                           (const uint32_t *)h->offtile);
+    }
     LAUNCHCHK(h, "k_p2g_tiled");
     rec(h, 1);
-    hipLaunchKernelGGL(k_p2g_offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, P, h->lines(), h->offtile,
-                       h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt);
-    LAUNCHCHK(h, "k_p2g_offtile");
+    if (!h->det()) {   // the deterministic tile kernel adds the off-tile points itself
+        hipLaunchKernelGGL(k_p2g_offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, P, h->lines(),
+                           h->offtile, h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt);
+        LAUNCHCHK(h, "k_p2g_offtile");
+    }
     rec(h, 2);
     return ICE_OK;
 }
@@ -665,6 +680,18 @@ int ice_set_tile(IceHandle *h, int tw, int th) {
     if (!find_tile(tw, th)) { h->err = "unsupported tile shape"; return ICE_E_ARG; }
     h->tile_w = tw;
     h->tile_h = th;
+    drop_graph(h);
+    return ICE_OK;
+}
+
+int ice_set_option(IceHandle *h, const char *name, int value) {
+    if (!h || !name) return ICE_E_ARG;
+    if (std::strcmp(name, "p2g_point_lanes") == 0) {
+        h->p2g_point_lanes = value != 0;
+    } else {
+        h->err = std::string("unknown option ") + name;
+        return ICE_E_ARG;
+    }
     drop_graph(h);
     return ICE_OK;
 }

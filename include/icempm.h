@@ -181,6 +181,10 @@ int ice_get_timings(IceHandle *h, double out[8]);
  * 64x32, 64x64.  No reference counterpart (the reference's only launch knobs
  * are tpb_P2G/Upd/G2P, parameters_sim.cpp:111-113). */
 int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
+/* Tuning switches of the sorted path (no reference counterpart):
+ * "p2g_point_lanes" = 1 maps one lane per point in the tiled P2G instead of
+ * one lane per cell (ignored in the deterministic mode). */
+int ice_set_option(IceHandle *h, const char *name, int value);
 
 /* ---- introspection */
 typedef struct IceInfo {

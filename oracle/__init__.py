@@ -51,6 +51,7 @@ def load():
         "orc_count": (C.c_uint32, [P]),
         "orc_get_grid": (C.c_int, [P, P, P, P]),
         "orc_clear_forces": (C.c_int, [P]),
+        "orc_set_grid_state": (C.c_int, [P, P, P, P, P, P]),
         "orc_error": (C.c_uint32, [P]),
         "orc_disabled": (C.c_int, [P]),
         "orc_reset_grid": (C.c_int, [P]),
@@ -148,6 +149,10 @@ class Oracle:
         mpp = np.zeros((3, N)); vel = np.zeros((2, N)); frc = np.zeros((2, N))
         self.lib.orc_get_grid(self.h, _p(mpp), _p(vel), _p(frc))
         return mpp, vel, frc
+
+    def set_grid_state(self, m=None, px=None, py=None, vx=None, vy=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (m, px, py, vx, vy)]
+        self.lib.orc_set_grid_state(self.h, *[_p(a) for a in arrs])
 
     def clear_forces(self):
         self.lib.orc_clear_forces(self.h)

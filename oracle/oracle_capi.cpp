@@ -186,6 +186,23 @@ int orc_get_grid(void *h, double *mpp, double *vel, double *forces) {
     return 0;
 }
 
+// Overwrite the node masses/momenta (after P2G) or node velocities (after the
+// update), for phase-level tests.  Each [N] doubles or NULL.
+int orc_set_grid_state(void *h, const double *m, const double *px, const double *py, const double *vx,
+                       const double *vy) {
+    DISPATCH(h, {
+        size_t N = (size_t)s.gx * s.gy;
+        for (size_t i = 0; i < N; i++) {
+            if (m) s.gm[i] = m[i];
+            if (px) s.gpx[i] = px[i];
+            if (py) s.gpy[i] = py[i];
+            if (vx) s.gvx[i] = vx[i];
+            if (vy) s.gvy[i] = vy[i];
+        }
+    });
+    return 0;
+}
+
 int orc_clear_forces(void *h) {
     DISPATCH(h, s.clear_forces());
     return 0;

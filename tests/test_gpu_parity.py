@@ -62,7 +62,10 @@ def test_p2g_grid_and_node_velocity(mode):
     o = oracle.Oracle.from_scene(sc, 64)
     o.substep(0.0, False)
     rmpp, rvel, _ = o.grid()
-    for k, name in enumerate(["m", "px", "py"]):
+    # the naive update overwrites (px, py) with the node velocity, as the
+    # reference's update kernel does (kernels.cuh:173-174)
+    names = ["m"] if mode == "naive" else ["m", "px", "py"]
+    for k, name in enumerate(names):
         scale = np.abs(rmpp[k]).max()
         e = np.abs(mpp[k] - rmpp[k]).max() / scale
         _print(name, e)
@@ -123,27 +126,58 @@ def test_many_substeps_dynamic_with_crossings():
 
 def test_config1_200_substeps_graph():
     """Config 1 (256x128, 60 % ice, 0.1 Pa wind) for 200 substeps through
-    ice_run with HIP-graph replays, against the FP64 oracle."""
+    ice_run with HIP-graph replays.
+
+    Config 1's strain increments (dt * C ~ 1e-10 per substep) are below the
+    FP32 spacing of Fe near 1 (6e-8), so in FP32 the ice drifts rigidly while
+    the FP64 oracle accumulates ~20 Pa of elastic stress near the sticky left
+    wall.  The HIP path is therefore held to the FP32 oracle field by field,
+    and to the FP64 oracle on what FP32 resolves: identical flags and cells,
+    and the mean wind-driven drift velocity within 1 %.  Per-field FP64 errors
+    are printed."""
     _print("test_config1_200_substeps_graph")
     sc = scenes.config1()
     sim = Simulation(sc)
     sim.run(0, 200, graph=True)
     pts, forces, err, dis = sim.download()
-    o = oracle.Oracle.from_scene(sc, 64)
-    o.run(0, 200)
-    ref = o.points()
-    rep = compare_points(pts, ref, fields=[
-        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-5),
-        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-3),
-        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-6),
-        ("Jp_inv", [abi.IDX_JP_INV], 1e-6)])
-    _print(rep)
-    assert err == o.error == 0
-    assert dis == o.disabled
-    # wind-driven drift along +x: every ice point moves right, ~1/2 a t^2
-    v = by_point(pts)[abi.IDX_VELX]
-    assert np.median(v) > 0
+    o32 = oracle.Oracle.from_scene(sc, 32)
+    o32.run(0, 200)
+    rep = compare_points(pts, o32.points(), fields=[
+        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-6),
+        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-7),
+        ("Jp_inv", [abi.IDX_JP_INV], 1e-7)])
+    _print("vs FP32 oracle", rep)
+    o64 = oracle.Oracle.from_scene(sc, 64)
+    o64.run(0, 200)
+    ref = by_point(o64.points_f64())
+    g = by_point(pts)
+    np.testing.assert_array_equal(u32(g, abi.IDX_CELL), ref[abi.IDX_CELL].astype(np.uint32))
+    np.testing.assert_array_equal(u32(g, abi.IDX_UTILITY), ref[abi.IDX_UTILITY].astype(np.uint32))
+    for name, lines in [("pos", [abi.IDX_POSX, abi.IDX_POSX + 1]), ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1]),
+                        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)))]:
+        e, scale = field_errors(g, ref, lines)
+        _print(f"FP64 oracle error {name}: max {e.max():.3e} median {np.median(e):.3e} (scale {scale:.3e})")
+    mean_v = g[abi.IDX_VELX].astype(np.float64).mean()
+    assert mean_v > 0 and mean_v == pytest.approx(ref[abi.IDX_VELX].mean(), rel=1e-2)
+    assert err == o64.error == 0
+    assert dis == o64.disabled
     sim.close()
+
+
+@pytest.mark.parametrize("lanes", ["cell", "point"])
+def test_p2g_lane_mappings_agree(lanes):
+    _print(f"test_p2g_lane_mappings_agree {lanes}")
+    sc = scenes.dynamic_scene(seed=17, speed=500.0)
+    a = Simulation(sc, flags=sc.params.flags | abi.FLAG_NAIVE)
+    b = Simulation(sc)
+    b._chk(b.lib.ice_set_option(b.h, b"p2g_point_lanes", int(lanes == "point")), "ice_set_option")
+    for k in range(12):
+        a.substep(DT * k, k == 11)
+        b.substep(DT * k, k == 11)
+    compare_points(b.download()[0], a.download()[0], max_flag_mismatch=2e-3, max_bad_frac=2e-3)
+    a.close()
+    b.close()
 
 
 def test_graph_replay_equals_eager():

@@ -1,0 +1,425 @@
+"""Pin the oracle against closed forms, invariants and independent numerics.
+
+The reference ships no tests, golden vectors or fixtures for this path, and it
+cannot be compiled here (Eigen3 and nvcc are absent), so each oracle function
+is pinned by mathematics chosen so that a plausible slip (dropped term, wrong
+sign or index, transposed operand) fails: linear-elastic limits, rotation
+invariance, numpy's SVD, conservation, APIC affine exactness, and the yield
+surface written in its textbook form.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from icempm_2d_sp_amd import abi, scenes
+from icempm_2d_sp_amd.params import sea_ice_params
+
+PRM = sea_ice_params(32, 24)
+LAM, MU, KAPPA = PRM.lame()
+
+
+def rot(theta):
+    c, s = math.cos(theta), math.sin(theta)
+    return np.array([[c, -s], [s, c]])
+
+
+# ---- SimParams derived constants (parameters_sim.cpp:122-140) -----------------
+
+def test_lame_closed_form():
+    # E = 1e9, nu = 0.3: lambda = E nu/((1+nu)(1-2nu)), mu = E/(2(1+nu))
+    assert LAM == pytest.approx(5.76923e8, rel=1e-5)
+    assert MU == pytest.approx(3.84615e8, rel=1e-5)
+    assert KAPPA == pytest.approx(8.33333e8, rel=1e-5)
+    cp = PRM.to_c()
+    oracle.load().orc_params_derive(cp)
+    assert cp.mu == pytest.approx(MU, rel=1e-15)
+    assert cp.kappa == pytest.approx(KAPPA, rel=1e-15)
+    assert cp.vmax == pytest.approx(0.25 * 1.0 / 1e-4)
+    assert cp.dt_vol_Dpinv == pytest.approx(1e-4 * PRM.ParticleVolume * 4.0)
+    assert cp.UpdateEveryNthStep == 100
+
+
+# ---- weights (kernels.cuh:323-332) ----------------------------------------------
+
+@pytest.mark.parametrize("x", np.linspace(-0.5, 0.5, 11))
+def test_weights_moments(x):
+    w = oracle.weights(x)
+    off = np.array([-1.0, 0.0, 1.0]) - x
+    assert w.sum() == pytest.approx(1.0, abs=1e-15)                 # partition of unity
+    assert (w * off).sum() == pytest.approx(0.0, abs=1e-15)         # first moment
+    assert (w * off ** 2).sum() == pytest.approx(0.25, abs=1e-15)   # D_p = h^2/4  ->  Dp_inv = 4/h^2
+    assert np.all(w >= 0)
+
+
+# ---- Kirchhoff stress (kernels.cuh:525-535) -------------------------------------
+
+def test_kirchhoff_identity_and_dilation():
+    assert np.abs(oracle.kirchhoff(PRM, np.eye(2))).max() == 0.0
+    s = 1.001
+    tau = oracle.kirchhoff(PRM, s * np.eye(2))
+    np.testing.assert_allclose(tau, 0.5 * KAPPA * (s ** 4 - 1) * np.eye(2), rtol=1e-12)
+
+
+def test_kirchhoff_small_strain_is_hooke():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        e = 1e-6 * rng.standard_normal((2, 2))
+        e = 0.5 * (e + e.T)
+        tau = oracle.kirchhoff(PRM, np.eye(2) + e)
+        hooke = 2 * MU * (e - 0.5 * np.trace(e) * np.eye(2)) + KAPPA * np.trace(e) * np.eye(2)
+        np.testing.assert_allclose(tau, hooke, atol=1e-4 * np.abs(hooke).max())
+
+
+def test_kirchhoff_rotation_covariance():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        F = np.eye(2) + 0.1 * rng.standard_normal((2, 2))
+        R = rot(rng.uniform(-3, 3))
+        np.testing.assert_allclose(oracle.kirchhoff(PRM, R @ F), R @ oracle.kirchhoff(PRM, F) @ R.T,
+                                   rtol=1e-10, atol=1e-3)
+
+
+# ---- ComputePQ (kernels.cuh:337-343) ----------------------------------------------
+
+def test_pq_against_numpy_svd():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        F = np.eye(2) + 0.05 * rng.standard_normal((2, 2))
+        J, p, q = oracle.pq(PRM, F)
+        s = np.linalg.svd(F, compute_uv=False)
+        Jn = np.linalg.det(F)
+        assert J == pytest.approx(Jn, rel=1e-13)
+        assert p == pytest.approx(-KAPPA / 2 * (Jn ** 2 - 1), rel=1e-9)
+        assert q == pytest.approx(MU * abs(s[0] ** 2 - s[1] ** 2) / Jn, rel=1e-9)
+
+
+# ---- SVD chain (kernels.cuh:486-508, helper_math.cuh:24-143, givens.cuh) ----------
+
+def _check_svd(F):
+    U, S, V = oracle.svd(F)
+    np.testing.assert_allclose(U @ np.diag(S) @ V.T, F, atol=1e-14 * max(1, np.abs(F).max()))
+    for M in (U, V):
+        np.testing.assert_allclose(M.T @ M, np.eye(2), atol=1e-14)
+        assert np.linalg.det(M) == pytest.approx(1.0, abs=1e-14)
+    assert S[0] >= S[1]
+    np.testing.assert_allclose(np.abs(S)[::-1] if abs(S[1]) > abs(S[0]) else np.abs(S),
+                               np.linalg.svd(F, compute_uv=False), rtol=1e-12, atol=1e-15)
+    return U, S, V
+
+
+def test_svd_random_and_inverted():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        F = rng.standard_normal((2, 2))
+        U, S, V = _check_svd(F)
+        if np.linalg.det(F) < 0:
+            assert S[1] < 0 <= S[0]      # the sign sits on the smaller singular value
+
+
+def test_svd_special_cases():
+    _check_svd(np.diag([2.0, 0.5]))             # y^2 == 0 branch (helper_math.cuh:93-99)
+    _check_svd(np.diag([0.5, 2.0]))             # needs the swap
+    U, S, V = oracle.svd(np.zeros((2, 2)))
+    assert np.all(S == 0) and np.allclose(U, np.eye(2)) and np.allclose(V, np.eye(2))
+    U, S, V = _check_svd(rot(0.7))
+    np.testing.assert_allclose(S, [1.0, 1.0], atol=1e-15)
+    np.testing.assert_allclose(U @ V.T, rot(0.7), atol=1e-15)
+
+
+# ---- failure surface (kernels.cuh:349-369) ----------------------------------------
+
+def _strength():
+    pmax, pmin, qmax, pmin2 = PRM.IceCompressiveStrength, -PRM.IceTensileStrength, PRM.IceShearStrength, \
+        -PRM.IceTensileStrength2
+    return pmax, pmin, qmax, pmin2
+
+
+def test_failure_surface_boundaries():
+    pmax, pmin, qmax, pmin2 = _strength()
+    # below the tensile cutoff
+    assert oracle.failure_check(PRM, 0, pmin2 * 1.01, 0.0) & abi.STATUS_CRUSHED
+    assert not oracle.failure_check(PRM, 0, pmin2 * 0.99, 0.0) & abi.STATUS_CRUSHED
+    # tension line through (pmin2, 0) and (0, q0), q0 = 2 sqrt(-pmax pmin) qmax / (pmax - pmin)
+    q0 = 2 * math.sqrt(-pmax * pmin) * qmax / (pmax - pmin)
+    p = 0.5 * pmin2
+    q_line = q0 * (1 - p / pmin2)
+    assert oracle.failure_check(PRM, 0, p, q_line * 1.001) & abi.STATUS_CRUSHED
+    assert not oracle.failure_check(PRM, 0, p, q_line * 0.999) & abi.STATUS_CRUSHED
+    # compressive ellipse (1+2b) q^2 + M^2 (p + b pmax)(p - pmax) = 0, b = |pmin|/pmax
+    b = -pmin / pmax
+    M2 = 4 * qmax ** 2 * (1 + 2 * b) / (pmax - pmin) ** 2
+    for p in (0.0, 0.3 * pmax, 0.9 * pmax):
+        qe = math.sqrt(-M2 * (p + b * pmax) * (p - pmax) / (1 + 2 * b))
+        assert oracle.failure_check(PRM, 0, p, qe * 1.001) & abi.STATUS_CRUSHED
+        assert not oracle.failure_check(PRM, 0, p, qe * 0.999) & abi.STATUS_CRUSHED
+    assert oracle.failure_check(PRM, 0, 1.01 * pmax, 0.0) & abi.STATUS_CRUSHED
+    # flags latch: other bits pass through
+    assert oracle.failure_check(PRM, 0x1234, 0.0, 0.0) == 0x1234
+
+
+# ---- return map (kernels.cuh:374-483) -----------------------------------------------
+
+def _polar_R(F):
+    U, s, Vt = np.linalg.svd(F)
+    R = U @ Vt
+    if np.linalg.det(R) < 0:
+        U[:, 1] *= -1
+        R = U @ Vt
+    return R
+
+
+def _cohesion():
+    return -PRM.DP_threshold_p * math.tan(math.radians(PRM.DP_phi))
+
+
+def test_return_map_tension_cases():
+    c = _cohesion()
+    dpt = PRM.DP_threshold_p
+    # isotropic stretch with p < DP apex: tension
+    s = math.sqrt(math.sqrt(1 - 2 * 1.5 * dpt / KAPPA))    # p = 1.5 * dpt
+    F = rot(0.3) @ (s * np.eye(2)) @ np.diag([1.0002, 0.9999])
+    J = np.linalg.det(F)
+    # case 1: Jp >= 0.1 -> F = polar rotation, Jp /= J
+    case, Fn, jp, u, p, q, err = oracle.constitutive(PRM, F, 0.5, abi.STATUS_CRUSHED)
+    assert case == 1 and p < dpt
+    np.testing.assert_allclose(Fn, _polar_R(F), atol=1e-14)
+    assert jp == pytest.approx(0.5 / J, rel=1e-14)
+    # case 0: Jp < 0.1 -> F = sqrt(J) R, Jp unchanged
+    case, Fn, jp, u, p, q, err = oracle.constitutive(PRM, F, 0.05, abi.STATUS_CRUSHED)
+    assert case == 0
+    np.testing.assert_allclose(Fn, math.sqrt(J) * _polar_R(F), atol=1e-14)
+    assert jp == 0.05
+    assert c == pytest.approx(1e5)
+
+
+def test_return_map_elastic_and_uncrushed():
+    F = np.eye(2) + np.array([[1e-6, 2e-6], [0.0, -1e-6]])
+    case, Fn, jp, u, p, q, err = oracle.constitutive(PRM, F, 1.0, abi.STATUS_CRUSHED)
+    assert case == 2 and np.array_equal(Fn, F) and jp == 1.0
+    case, Fn, jp, u, p, q, err = oracle.constitutive(PRM, F, 1.0, 0)
+    assert case == -1 and u == 0 and np.array_equal(Fn, F)
+
+
+def test_return_map_plastic_lands_on_yield_surface():
+    """Case 3: the projected (p, q) lies on min(ellipse, Drucker-Prager with
+    cohesion c: q = c + p tan(phi)), p is capped by the ridge pressure, and the
+    singular vectors are kept."""
+    rng = np.random.default_rng(4)
+    pmax, pmin, qmax, _ = _strength()
+    c, tphi = _cohesion(), math.tan(math.radians(PRM.DP_phi))
+    n3 = 0
+    for _ in range(200):
+        F = rot(rng.uniform(-3, 3)) @ np.diag(1 + 2e-3 * rng.standard_normal(2)) @ rot(rng.uniform(-3, 3))
+        Jp = rng.uniform(0.2, 1.5)
+        th = rng.uniform(0.5, 2.0)
+        case, Fn, jp, u, p, q, err = oracle.constitutive(PRM, F, Jp, abi.STATUS_CRUSHED, thickness=th)
+        if case != 3:
+            continue
+        n3 += 1
+        J = np.linalg.det(F)
+        ridge = PRM.RidgeFormationCoeff * 9.8 * PRM.IceDensity * th * Jp
+        p1 = min(p, ridge)
+        q1 = min(2 * math.sqrt((pmax - p1) * (p1 - pmin)) * qmax / (pmax - pmin), max(0.0, c + p1 * tphi))
+        s_new = np.linalg.svd(Fn, compute_uv=False)
+        Jn = np.prod(s_new) * np.sign(np.linalg.det(Fn))
+        # mean squared stretch encodes p1: -kappa/2 (mean(s^2)^2 - 1) = p1
+        m2 = 0.5 * (s_new ** 2).sum()
+        assert -KAPPA / 2 * (m2 ** 2 - 1) == pytest.approx(p1, rel=1e-6, abs=1e-3)
+        # deviatoric part: q of the new state scaled by J_new/J_trial equals q1
+        assert MU * abs(s_new[0] ** 2 - s_new[1] ** 2) / abs(J) == pytest.approx(q1, rel=1e-6, abs=1e-6)
+        assert jp == pytest.approx(Jp * math.sqrt(1 - 2 * p1 / KAPPA) / J, rel=1e-12)
+        # singular vectors unchanged: Fn F^-1 ... the polar rotation is preserved
+        np.testing.assert_allclose(_polar_R(Fn), _polar_R(F), atol=1e-9)
+    assert n3 > 20
+
+
+# ---- grid phases ------------------------------------------------------------------
+
+def _tiny_scene(seed=0, n_side=6):
+    sc = scenes.dynamic_scene(seed=seed, gx=n_side + 6, gy=n_side + 5, fill=0.6)
+    return sc
+
+
+def test_p2g_conserves_mass_and_momentum():
+    sc = _tiny_scene()
+    o = oracle.Oracle.from_scene(sc, 64)
+    o.reset_grid()
+    o.p2g()
+    mpp, _, _ = o.grid()
+    pts = sc.soa.astype(np.float64)
+    mp = sc.params.ParticleMass * pts[abi.IDX_THICKNESS]
+    assert mpp[0].sum() == pytest.approx(mp.sum(), rel=1e-13)
+    assert mpp[1].sum() == pytest.approx((mp * pts[abi.IDX_VELX]).sum(), rel=1e-11)
+    assert mpp[2].sum() == pytest.approx((mp * pts[abi.IDX_VELX + 1]).sum(), rel=1e-11)
+
+
+def test_p2g_single_point_brute_force():
+    """One point: node (i, j) gets w_i(x) w_j(y) m and w (m v + S (x_i - x_p))."""
+    sc = _tiny_scene()
+    soa = sc.soa[:, :1].copy()
+    o = oracle.Oracle(sc.params, 64)
+    o.set_points(soa)
+    o.set_grid(sc.status)
+    o.reset_grid()
+    o.p2g()
+    mpp, _, _ = o.grid()
+    gy = sc.params.GridYTotal
+    cell = int(soa[abi.IDX_CELL].view(np.uint32)[0])
+    cx, cy = cell & 0xFFFF, cell >> 16
+    x, y = float(soa[abi.IDX_POSX, 0]), float(soa[abi.IDX_POSX + 1, 0])
+    m = sc.params.ParticleMass * float(soa[abi.IDX_THICKNESS, 0])
+    nz = np.nonzero(mpp[0])[0]
+    assert len(nz) == 9
+    for a in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            node = (cy + b) + (cx + a) * gy
+            w = oracle.weights(x)[a + 1] * oracle.weights(y)[b + 1]
+            assert mpp[0, node] == pytest.approx(w * m, rel=1e-14)
+
+
+def test_g2p_reproduces_affine_velocity_field():
+    """APIC with quadratic B-splines is exact for v(x) = a + B x."""
+    sc = _tiny_scene(seed=1)
+    o = oracle.Oracle.from_scene(sc, 64)
+    gx, gy = sc.params.GridXTotal, sc.params.GridYTotal
+    X, Y = np.meshgrid(np.arange(gx, dtype=float), np.arange(gy, dtype=float), indexing="ij")
+    a = np.array([0.3, -0.2])
+    B = np.array([[0.01, -0.02], [0.015, 0.005]])
+    vx = (a[0] + B[0, 0] * X + B[0, 1] * Y).reshape(-1)
+    vy = (a[1] + B[1, 0] * X + B[1, 1] * Y).reshape(-1)
+    o.set_grid_state(vx=vx, vy=vy)
+    o.g2p(False)
+    out = o.points_f64()
+    pts = sc.soa.astype(np.float64)
+    cell = sc.soa[abi.IDX_CELL].view(np.uint32)
+    xp = (cell & 0xFFFF) + pts[abi.IDX_POSX]
+    yp = (cell >> 16) + pts[abi.IDX_POSX + 1]
+    np.testing.assert_allclose(out[abi.IDX_VELX], a[0] + B[0, 0] * xp + B[0, 1] * yp, atol=1e-13)
+    np.testing.assert_allclose(out[abi.IDX_VELX + 1], a[1] + B[1, 0] * xp + B[1, 1] * yp, atol=1e-13)
+    for k, (i, j) in enumerate([(0, 0), (0, 1), (1, 0), (1, 1)]):
+        np.testing.assert_allclose(out[abi.IDX_BP00 + k], B[i, j], atol=1e-13)   # C = B (h = 1)
+
+
+def test_update_nodes_closed_forms():
+    prm = sea_ice_params(8, 8).replace(waterDragEffectiveLinear=0.2)
+    prm.ParticleVolume = 0.25
+    o = oracle.Oracle(prm, 64)
+    N = 64
+    status = np.full(N, 100, np.uint8)
+    status[5] = 7                                # a wall node
+    cur = np.zeros((2, N), np.float32)
+    cur[0, 3] = 2.0
+    wind = np.zeros((2, N), np.float32)
+    wind[1, 4] = 0.5
+    o.set_points(np.zeros((19, 0), np.float32))
+    o.set_grid(status, cur, wind)
+    m = np.zeros(N); px = np.zeros(N); py = np.zeros(N)
+    m[1], px[1], py[1] = 2.0, 4.0, -2.0          # plain node, no current: drag pulls toward 0
+    m[3], px[3] = 1.0, 1.0                       # current 2 m/s ramped by (1 + t/7200)
+    m[4] = 1.0                                   # wind only
+    m[5], px[5], py[5] = 1.0, 3.0, 1.0           # wall
+    m[6], px[6] = 1.0, 2000.0                    # above the clamp |v|^2 > 0.1 vmax^2
+    px[7] = 5.0                                  # m == 0: untouched
+    o.set_grid_state(m=m, px=px, py=py)
+    t = 360.0
+    o.update_nodes(t)
+    _, vel, frc = o.grid()
+    dt = prm.InitialTimeStep
+    kL, kQ = 0.2 * dt, prm.waterDragEffectiveQuadratic * dt
+    v = np.array([2.0, -1.0]); u = -v
+    k = min(kL + kQ * np.linalg.norm(u), 0.1)
+    np.testing.assert_allclose(vel[:, 1], v + k * u, rtol=1e-14)
+    w = 2.0 * (1 + t / 7200.0)
+    u3 = np.array([w - 1.0, 0.0])
+    k3 = min(kL + kQ * np.linalg.norm(u3), 0.1)
+    np.testing.assert_allclose(vel[:, 3], np.array([1.0, 0.0]) + k3 * u3, rtol=1e-14)
+    np.testing.assert_allclose(vel[:, 4], [0.0, dt * 0.5 * prm.windStressToAccel], rtol=1e-14)
+    assert np.all(vel[:, 5] == 0) and frc[0, 5] == 3.0 and frc[1, 5] == 1.0
+    assert np.all(vel[:, 6] == 0)
+    assert vel[0, 7] == 5.0 and vel[1, 7] == 0.0
+    assert np.count_nonzero(frc) == 2
+
+
+def test_update_drag_cap():
+    prm = sea_ice_params(8, 8).replace(waterDragEffectiveQuadratic=1e4)
+    o = oracle.Oracle(prm, 64)
+    o.set_points(np.zeros((19, 0), np.float32))
+    o.set_grid(np.full(64, 100, np.uint8))
+    m = np.zeros(64); px = np.zeros(64)
+    m[2], px[2] = 1.0, 10.0
+    o.set_grid_state(m=m, px=px)
+    o.update_nodes(0.0)
+    _, vel, _ = o.grid()
+    assert vel[0, 2] == pytest.approx(10.0 - 0.1 * 10.0)     # k capped at 0.1
+
+
+# ---- G2P cell crossing and disable band (kernels.cuh:248-265) --------------------
+
+def test_g2p_cell_crossing_and_disable():
+    prm = sea_ice_params(12, 12)
+    prm.ParticleVolume = 0.25
+    soa = scenes.make_points(np.array([2.0, 5.0]), np.array([6.0, 5.0]), 12, 12)
+    soa = soa[:, ::-1].copy()                   # point 0 at cell x = 5, point 1 at cell x = 2
+    assert list(soa[abi.IDX_CELL].view(np.uint32) & 0xFFFF) == [5, 2]
+    soa[abi.IDX_POSX] = [0.49, -0.49]
+    o = oracle.Oracle(prm, 64)
+    o.set_points(soa)
+    o.set_grid(np.full(144, 100, np.uint8))
+    N = 144
+    o.set_grid_state(vx=np.full(N, 0.02 / prm.InitialTimeStep * prm.cellsize),
+                     vy=np.zeros(N))
+    o.g2p(False)
+    out = o.points()
+    cell = out[abi.IDX_CELL].view(np.uint32)
+    util = out[abi.IDX_UTILITY].view(np.uint32)
+    # first point: 0.49 + 0.02 > 0.5 -> next cell, x -= 1
+    assert (cell[0] & 0xFFFF) == 6 and out[abi.IDX_POSX, 0] == pytest.approx(-0.49, abs=1e-6)
+    assert not util[0] & abi.STATUS_DISABLED
+    # second point at cell x = 2 moving right: -0.47, stays
+    assert (cell[1] & 0xFFFF) == 2
+    # now push the second point left into the band x <= 1
+    o2 = oracle.Oracle(prm, 64)
+    o2.set_points(soa)
+    o2.set_grid(np.full(144, 100, np.uint8))
+    o2.set_grid_state(vx=np.full(N, -0.02 / prm.InitialTimeStep), vy=np.zeros(N))
+    o2.g2p(False)
+    out2 = o2.points()
+    assert out2[abi.IDX_UTILITY].view(np.uint32)[1] & abi.STATUS_DISABLED
+    assert o2.disabled == 1
+
+
+# ---- sort (host_side_soa.cpp:36-47) ------------------------------------------------
+
+def test_sort_is_stable_sort_by_cell_index():
+    rng = np.random.default_rng(5)
+    sc = scenes.dynamic_scene(seed=6)
+    soa = sc.soa.copy()
+    perm0 = rng.permutation(soa.shape[1])
+    soa = soa[:, perm0]
+    util = soa[abi.IDX_UTILITY].view(np.uint32)
+    util[rng.random(soa.shape[1]) < 0.1] |= abi.STATUS_DISABLED
+    o = oracle.Oracle(sc.params, 64)
+    o.set_points(soa)
+    perm = o.sort()
+    cell = soa[abi.IDX_CELL].view(np.uint32)
+    key = (cell & 0xFFFF).astype(np.int64) * sc.params.GridYTotal + (cell >> 16)
+    keep = np.nonzero((util & abi.STATUS_DISABLED) == 0)[0]
+    expect = keep[np.argsort(key[keep], kind="stable")]
+    np.testing.assert_array_equal(perm, expect)
+    assert o.n == keep.size
+
+
+def test_fp32_oracle_tracks_fp64():
+    """The FP32 restatement stays within FP32 rounding of the FP64 one on one
+    substep (this is the precision floor a plain FP32 port would have)."""
+    sc = scenes.dynamic_scene(seed=2)
+    a = oracle.Oracle.from_scene(sc, 64)
+    b = oracle.Oracle.from_scene(sc, 32)
+    a.substep(0.0, True)
+    b.substep(0.0, True)
+    pa, pb = a.points_f64(), b.points_f64()
+    for line in (abi.IDX_POSX, abi.IDX_VELX, abi.IDX_FE00):
+        scale = np.abs(pa[line]).max()
+        assert np.abs(pa[line] - pb[line]).max() / scale < 1e-5
