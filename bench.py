@@ -71,6 +71,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--naive", action="store_true", help="reference launch structure (unsorted, atomics)")
     ap.add_argument("--scale", type=float, default=1.0, help="linear domain scale (1.0 = 4096x2048)")
+    ap.add_argument("--copy-only", type=float, default=0.0,
+                    help="only time a plain copy of this many GB (HBM reference / counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -80,6 +82,16 @@ def main() -> int:
 
     from icempm_2d_sp_amd import abi, scenes
     from icempm_2d_sp_amd.gpu import Simulation
+
+    if args.copy_only > 0:
+        import ctypes as C
+        nbytes = int(args.copy_only * 1e9)
+        ms = C.c_double(0)
+        abi.check(abi.load().ice_copy_bench(0, nbytes, 20, C.byref(ms)), None, "ice_copy_bench")
+        gbs = 2 * nbytes / (ms.value * 1e-3) / 1e9
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms.value, "GBs_read_plus_write": gbs,
+                          "frac_of_peak": gbs / HBM_PEAK_GBS}), flush=True)
+        return 0
 
     gx, gy = int(round(4096 * args.scale)), int(round(2048 * args.scale))
     t0 = time.time()

@@ -95,6 +95,7 @@ SIGNATURES = {
     "ice_set_tile": (C.c_int, [_P, C.c_int, C.c_int]),
     "ice_set_option": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "ice_get_info": (C.c_int, [_P, C.POINTER(IceInfo)]),
+    "ice_copy_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -242,6 +242,8 @@ int scan_u32(IceHandle *h, const uint32_t *in, size_t n, uint32_t *out) {
     return ICE_OK;
 }
 
+void drop_graph(IceHandle *h);
+
 // Sort points into cell order; keep disabled points behind (drop = false) or
 // drop them (drop = true).  Leaves cs valid and the off-tile list empty.
 int device_sort(IceHandle *h, bool drop, uint32_t *perm_dev) {
@@ -276,6 +278,7 @@ int device_sort(IceHandle *h, bool drop, uint32_t *perm_dev) {
     }
     std::swap(h->pts, h->pts_alt);
     h->n = n_out;
+    drop_graph(h);   // a captured graph holds the old point buffer
     if (drop) {
         const int32_t zero = 0;
         HIPCHK(h, hipMemcpyAsync(&h->cnt->disabled, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
@@ -899,6 +902,39 @@ int ice_get_timings(IceHandle *h, double out[8]) {
     h->ev_used = 0;
     for (int k = 0; k < 8; k++) out[k] = h->acc[k];
     return ICE_OK;
+}
+
+int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
+    if (!ms_per_copy || reps <= 0 || bytes < 1024) return ICE_E_ARG;
+    *ms_per_copy = 0.0;
+    if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice failed"; return ICE_E_DEVICE; }
+    const size_t n4 = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ICE_OK;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && rc == ICE_OK) { g_create_error = hipGetErrorString(e); rc = ICE_E_HIP; }
+        return rc == ICE_OK;
+    };
+    if (ok(hipMalloc(&a, n4 * 16)) && ok(hipMalloc(&b, n4 * 16)) && ok(hipStreamCreate(&st)) &&
+        ok(hipEventCreate(&e0)) && ok(hipEventCreate(&e1)) && ok(hipMemsetAsync(a, 0, n4 * 16, st))) {
+        const unsigned blocks = 4096;
+        hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, st, a, b, n4);   // warm-up
+        ok(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, st, a, b, n4);
+        ok(hipGetLastError());
+        ok(hipEventRecord(e1, st));
+        float ms = 0.f;
+        if (ok(hipEventSynchronize(e1)) && ok(hipEventElapsedTime(&ms, e0, e1))) *ms_per_copy = ms / reps;
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int ice_get_info(IceHandle *h, IceInfo *info) {

@@ -18,6 +18,13 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *__restrict__ a, size
         a[i] = v;
 }
 
+// Plain device copy: the reference point for HBM bandwidth and for scaling
+// the profiler's FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy4(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 __global__ void k_step_advance(Counters *cnt, int by) {
     if (threadIdx.x == 0 && blockIdx.x == 0) cnt->step += by;
 }

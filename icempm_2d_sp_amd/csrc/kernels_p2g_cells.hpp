@@ -4,9 +4,10 @@
 // into cells T +- 2 are pulled, every node of T is written once by its tile),
 // but lanes map to cells, not points: a lane walks the points sorted into its
 // cell, sums their 3x3 stencil contributions in registers, and issues 27 LDS
-// atomics per cell.  Consecutive lanes own consecutive cells, so one atomic
-// instruction never has two lanes on the same node, and there are ~4x fewer
-// atomics than with one lane per point (4 points per cell).
+// atomics per cell (deterministic mode) or, by default, adds them in nine
+// barrier-separated phases of plain LDS read-add-write (one phase per stencil
+// offset: within a phase all cells of a round hit distinct nodes).  No LDS
+// atomics remain on the common path and the float sum order is fixed.
 //
 // Points that changed cell since the sort (kMoved) may contribute around a
 // cell next to their lane's; they add their own stencil directly.  In the
@@ -54,13 +55,78 @@ __device__ __forceinline__ void scatter_point(const P2GPoint &q, int x0, int y0,
         }
 }
 
+// Sum the unmoved points of cell (x, y) into the register stencil; returns
+// whether any point contributed and sets *moved when the cell also holds
+// points that changed cell since the sort.
+__device__ __forceinline__ bool cell_stencil(const Lines &L, const DevParams &P, uint32_t b, uint32_t e, int x, int y,
+                                             bool ring2, float am[3][3], float ax[3][3], float ay[3][3],
+                                             bool &moved, uint32_t &err) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) am[i][j] = ax[i][j] = ay[i][j] = 0.f;
+    bool any = false;
+    moved = false;
+    const bool inside = x >= 1 && y >= 1 && x <= P.gx - 2 && y <= P.gy - 2;
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t u = L.u(ICE_IDX_UTILITY, i);
+        if (u & (kDisabled | kEscaped)) continue;
+        if (u & kMoved) { moved = true; continue; }
+        if (ring2) continue;   // an unmoved point two cells out cannot reach the tile
+        if (!inside) { err |= ICE_ERR_POINT_LEFT_AREA; continue; }
+        P2GPoint q;
+        load_p2g_point(L, i, ((uint32_t)y << 16) | (uint32_t)x, P, q);
+        float wx[3], wy[3];
+        weights(q.x, wx);
+        weights(q.y, wy);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++) {
+                float dm, dpx, dpy;
+                p2g_contrib(q, a - 1, bb - 1, wx[a] * wy[bb], dm, dpx, dpy);
+                am[a][bb] += dm;
+                ax[a][bb] += dpx;
+                ay[a][bb] += dpy;
+            }
+        any = true;
+    }
+    if (any) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++) {
+                if (__builtin_isnan(ax[a][bb]) || __builtin_isnan(ay[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+                if (__builtin_isnan(am[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+            }
+    }
+    return any;
+}
+
+// Moved points of cell [b, e): their own stencil, added with LDS atomics.
+template <int TW, int TH, int LP, bool DET>
+__device__ __forceinline__ void scatter_moved(const Lines &L, const DevParams &P, uint32_t b, uint32_t e, int x0, int y0,
+                                              typename std::conditional<DET, unsigned long long, float>::type *acc,
+                                              const FixScale &fs, uint32_t &err) {
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t u = L.u(ICE_IDX_UTILITY, i);
+        if ((u & (kDisabled | kEscaped)) || !(u & kMoved)) continue;
+        const uint32_t cw = L.u(ICE_IDX_CELL, i);
+        P2GPoint q;
+        load_p2g_point(L, i, cw, P, q);
+        if (!stencil_inside(q.cx, q.cy, P)) { err |= ICE_ERR_POINT_LEFT_AREA; continue; }
+        scatter_point<TW, TH, LP, DET>(q, x0, y0, acc, fs, err);
+    }
+}
+
 template <int TW, int TH, bool DET>
 __global__ __launch_bounds__(kTileThreads) void k_p2g_cells(DevParams P, Lines L, const uint32_t *__restrict__ cs,
                                                             float *__restrict__ gm, float *__restrict__ gpx,
                                                             float *__restrict__ gpy, Counters *cnt, int ntx, int nty,
                                                             FixScale fs, const uint32_t *__restrict__ offtile) {
     constexpr int RW = TW + 4, RH = TH + 4;   // cells pulled
-    constexpr int LP = TH + 1;                // padded LDS column pitch (no 2-way bank conflicts)
+    constexpr int NCELL = RW * RH;
+    constexpr int LP = TH + 1;                // padded LDS column pitch
     constexpr int NN = TW * LP;
     using Acc = typename std::conditional<DET, unsigned long long, float>::type;
     __shared__ Acc s_acc[3 * NN];
@@ -72,61 +138,54 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_cells(DevParams P, Lines L
     __syncthreads();
 
     uint32_t err = 0;
-    for (int k = threadIdx.x; k < RW * RH; k += kTileThreads) {
+    // Rounds of kTileThreads cells; the loop bound is uniform so every thread
+    // reaches every barrier.
+    for (int base = 0; base < NCELL; base += kTileThreads) {
+        const int k = base + (int)threadIdx.x;
         const int c = k / RH, r = k - (k / RH) * RH;
         const int x = x0 - 2 + c, y = y0 - 2 + r;
-        if (x < 0 || x >= P.gx || y < 0 || y >= P.gy) continue;
-        const bool ring2 = (c == 0 || c == RW - 1 || r == 0 || r == RH - 1);
-        const size_t cell = (size_t)x * P.gy + y;
-        const uint32_t b = cs[cell], e = cs[cell + 1];
-        float am[3][3], ax[3][3], ay[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) am[i][j] = ax[i][j] = ay[i][j] = 0.f;
-        bool any = false;
-        const bool inside = x >= 1 && y >= 1 && x <= P.gx - 2 && y <= P.gy - 2;
-        for (uint32_t i = b; i < e; i++) {
-            const uint32_t u = L.u(ICE_IDX_UTILITY, i);
-            if (u & (kDisabled | kEscaped)) continue;
-            if (u & kMoved) {
-                const uint32_t cw = L.u(ICE_IDX_CELL, i);
-                P2GPoint q;
-                load_p2g_point(L, i, cw, P, q);
-                if (!stencil_inside(q.cx, q.cy, P)) { err |= ICE_ERR_POINT_LEFT_AREA; continue; }
-                scatter_point<TW, TH, LP, DET>(q, x0, y0, s_acc, fs, err);
-                continue;
-            }
-            if (ring2) continue;   // an unmoved point two cells out cannot reach T
-            if (!inside) { err |= ICE_ERR_POINT_LEFT_AREA; continue; }
-            P2GPoint q;
-            load_p2g_point(L, i, ((uint32_t)y << 16) | (uint32_t)x, P, q);
-            float wx[3], wy[3];
-            weights(q.x, wx);
-            weights(q.y, wy);
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int bb = 0; bb < 3; bb++) {
-                    float dm, dpx, dpy;
-                    p2g_contrib(q, a - 1, bb - 1, wx[a] * wy[bb], dm, dpx, dpy);
-                    am[a][bb] += dm;
-                    ax[a][bb] += dpx;
-                    ay[a][bb] += dpy;
-                }
-            any = true;
+        const bool valid = k < NCELL && x >= 0 && x < P.gx && y >= 0 && y < P.gy;
+        uint32_t b = 0, e = 0;
+        if (valid) {
+            const size_t cell = (size_t)x * P.gy + y;
+            b = cs[cell];
+            e = cs[cell + 1];
         }
-        if (any) {
+        const bool ring2 = (c == 0 || c == RW - 1 || r == 0 || r == RH - 1);
+        float am[3][3], ax[3][3], ay[3][3];
+        bool moved = false;
+        const bool any = valid && cell_stencil(L, P, b, e, x, y, ring2, am, ax, ay, moved, err);
+        if constexpr (DET) {
+            if (any) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int bb = 0; bb < 3; bb++) {
+                        const int lx = x + a - 1 - x0, ly = y + bb - 1 - y0;
+                        if ((unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH)
+                            lds_add3<DET, NN>(s_acc, lx * LP + ly, am[a][bb], ax[a][bb], ay[a][bb], fs);
+                    }
+            }
+            if (moved) scatter_moved<TW, TH, LP, DET>(L, P, b, e, x0, y0, s_acc, fs, err);
+        } else {
+            // Nine phases, one per stencil offset: inside a phase every cell of
+            // the round targets a different node, so plain LDS read-add-write
+            // suffices, and the sum order is fixed.
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int bb = 0; bb < 3; bb++) {
-                    if (__builtin_isnan(ax[a][bb]) || __builtin_isnan(ay[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-                    if (__builtin_isnan(am[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+                    __syncthreads();
                     const int lx = x + a - 1 - x0, ly = y + bb - 1 - y0;
-                    if ((unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH)
-                        lds_add3<DET, NN>(s_acc, lx * LP + ly, am[a][bb], ax[a][bb], ay[a][bb], fs);
+                    if (any && (unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH) {
+                        const int o = lx * LP + ly;
+                        s_acc[o] += am[a][bb];
+                        s_acc[NN + o] += ax[a][bb];
+                        s_acc[2 * NN + o] += ay[a][bb];
+                    }
                 }
+            __syncthreads();
+            if (moved) scatter_moved<TW, TH, LP, DET>(L, P, b, e, x0, y0, s_acc, fs, err);
         }
     }
     if constexpr (DET) {

@@ -203,6 +203,11 @@ typedef struct IceInfo {
 } IceInfo;
 int ice_get_info(IceHandle *h, IceInfo *info);
 
+/* Measurement utility (no reference counterpart): average time of a plain
+ * device-to-device copy of `bytes` (read + write), over `reps` launches on
+ * `device`.  Used to read HBM bandwidth and to scale profiler byte counters. */
+int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
