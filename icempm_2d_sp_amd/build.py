@@ -54,7 +54,26 @@ def build_library(verbose: bool = False) -> str:
     if verbose and res.stderr.strip():
         print(res.stderr, file=sys.stderr)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER = os.path.join(LIBDIR, "model_driver")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The Model::Step-style C++ driver over the GPU_Implementation5 shim."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "model_driver.cpp"), "-L", LIBDIR, "-licempm", "-Wl,-rpath,$ORIGIN",
+           "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
