@@ -66,6 +66,7 @@ struct IceHandle {
     // tiles
     int tile_w = 32, tile_h = 32;
     int p2g_point_lanes = 0;       // 1: one lane per point in the tiled P2G (the default is one per cell)
+    int band = 4;                  // tile columns per band of the workgroup -> tile order
     FixScale fix{};
     // phase state
     double pending_t = 0.0;
@@ -203,9 +204,9 @@ int launch_fill_u32(IceHandle *h, uint32_t *a, size_t n, uint32_t v) {
 
 using P2GFn = void (*)(DevParams, Lines, const uint32_t *, float *, float *, float *, Counters *, int, int, FixScale);
 using P2GCellFn = void (*)(DevParams, Lines, const uint32_t *, float *, float *, float *, Counters *, int, int,
-                           FixScale, const uint32_t *);
+                           FixScale, const uint32_t *, int);
 using G2PFn = void (*)(DevParams, StepArg, Lines, const uint32_t *, GridIn, float *, float *, Counters *, uint32_t *,
-                       int, int);
+                       int, int, int);
 
 struct TileKernels {
     int w, h;
@@ -354,7 +355,7 @@ int phase_p2g(IceHandle *h) {
         P2GCellFn fn = h->det() ? tk->p2gc_det : tk->p2gc;
         hipLaunchKernelGGL(fn, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, h->lines(), h->s_cs,
                            h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt, ntx, nty, h->fix,
-                           (const uint32_t *)h->offtile);
+                           (const uint32_t *)h->offtile, h->band);
     }
     LAUNCHCHK(h, "k_p2g_tiled");
     rec(h, 1);
@@ -398,7 +399,7 @@ int phase_g2p(IceHandle *h, const StepArg &S) {
     const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
     const int ntx = (P.gx + h->tile_w - 1) / h->tile_w, nty = (P.gy + h->tile_h - 1) / h->tile_h;
     hipLaunchKernelGGL(tk->g2p, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, S, h->lines(), h->s_cs, G,
-                       h->gline(G_FX), h->gline(G_FY), h->cnt, h->offtile, ntx, nty);
+                       h->gline(G_FX), h->gline(G_FY), h->cnt, h->offtile, ntx, nty, h->band);
     LAUNCHCHK(h, "k_g2p_tiled");
     rec(h, 4);
     h->steps_since_sort++;
@@ -691,6 +692,9 @@ int ice_set_option(IceHandle *h, const char *name, int value) {
     if (!h || !name) return ICE_E_ARG;
     if (std::strcmp(name, "p2g_point_lanes") == 0) {
         h->p2g_point_lanes = value != 0;
+    } else if (std::strcmp(name, "band") == 0) {
+        if (value < 1 || value > 4096) { h->err = "band must be 1..4096"; return ICE_E_ARG; }
+        h->band = value;
     } else {
         h->err = std::string("unknown option ") + name;
         return ICE_E_ARG;

@@ -123,7 +123,8 @@ template <int TW, int TH, bool DET>
 __global__ __launch_bounds__(kTileThreads) void k_p2g_cells(DevParams P, Lines L, const uint32_t *__restrict__ cs,
                                                             float *__restrict__ gm, float *__restrict__ gpx,
                                                             float *__restrict__ gpy, Counters *cnt, int ntx, int nty,
-                                                            FixScale fs, const uint32_t *__restrict__ offtile) {
+                                                            FixScale fs, const uint32_t *__restrict__ offtile,
+                                                            int band) {
     constexpr int RW = TW + 4, RH = TH + 4;   // cells pulled
     constexpr int NCELL = RW * RH;
     constexpr int LP = TH + 1;                // padded LDS column pitch
@@ -131,8 +132,8 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_cells(DevParams P, Lines L
     using Acc = typename std::conditional<DET, unsigned long long, float>::type;
     __shared__ Acc s_acc[3 * NN];
 
-    const int t = xcd_remap(blockIdx.x, ntx * nty);
-    const int tx = t / nty, ty = t - tx * nty;
+    int tx, ty;
+    tile_coords(blockIdx.x, ntx, nty, band, tx, ty);
     const int x0 = tx * TW, y0 = ty * TH;
     for (int k = threadIdx.x; k < 3 * NN; k += kTileThreads) s_acc[k] = 0;
     __syncthreads();

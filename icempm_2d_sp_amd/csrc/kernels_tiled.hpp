@@ -35,6 +35,28 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
     return xcd * per + min(xcd, rem) + j;
 }
 
+// Tile of workgroup `bid`: XCD-contiguous runs (xcd_remap) of a band order
+// (bands of `band` tile columns, rows outer, columns inner), so the tiles in
+// flight on one XCD form a compact block whose x and y neighbours share halo
+// points and nodes in that XCD's L2.
+__device__ __forceinline__ void tile_coords(int bid, int ntx, int nty, int band, int &tx, int &ty) {
+    const int t = xcd_remap(bid, ntx * nty);
+    const int b = t / (band * nty);
+    const int wb = min(band, ntx - b * band);
+    const int r = t - b * band * nty;
+    ty = r / wb;
+    tx = b * band + (r - ty * wb);
+}
+
+// Column of flattened item k (last column whose prefix is <= k), searched
+// from a wave-uniform start column cb.
+template <int NCOL>
+__device__ __forceinline__ int col_from(uint32_t k, int cb, const uint32_t *s_pre) {
+    int c = cb;
+    while (c < NCOL - 1 && s_pre[c + 1] <= k) c++;
+    return c;
+}
+
 struct FixScale {
     float m, p;            // float -> fixed point
     double inv_m, inv_p;   // fixed point -> value
@@ -68,6 +90,18 @@ __device__ __forceinline__ void column_ranges(const uint32_t *__restrict__ cs, c
         s_pre[ncol] = acc;
     }
     __syncthreads();
+}
+
+// Column of flattened item k: the last column whose prefix is <= k.
+template <int NCOL>
+__device__ __forceinline__ int flat_col(uint32_t k, const uint32_t *s_pre) {
+    int lo = 0, hi = NCOL - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Slot of flattened item k: the last column whose prefix is <= k.
@@ -171,14 +205,14 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiled(DevParams P, StepArg
                                                             const uint32_t *__restrict__ cs, GridIn G,
                                                             float *__restrict__ gfx, float *__restrict__ gfy,
                                                             Counters *cnt, uint32_t *__restrict__ offtile, int ntx,
-                                                            int nty) {
+                                                            int nty, int band) {
     constexpr int RW = TW + 4, RH = TH + 4;
     __shared__ float2 s_v[RW * RH];
     __shared__ uint32_t s_rb[TW];
     __shared__ uint32_t s_pre[TW + 1];
 
-    const int t = xcd_remap(blockIdx.x, ntx * nty);
-    const int tx = t / nty, ty = t - tx * nty;
+    int tx, ty;
+    tile_coords(blockIdx.x, ntx, nty, band, tx, ty);
     const int x0 = tx * TW, y0 = ty * TH;
     double time;
     bool rec;
@@ -208,13 +242,21 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiled(DevParams P, StepArg
     column_ranges(cs, P, x0, TW, y0, y0 + TH, s_rb, s_pre);   // contains __syncthreads
 
     const uint32_t total = s_pre[TW];
-    for (uint32_t k = threadIdx.x; k < total; k += kTileThreads) {
-        const uint32_t i = flat_slot<TW>(k, s_rb, s_pre);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // each wave takes every fourth chunk of 64 consecutive items
+    for (uint32_t kb = (uint32_t)wave * 64; kb < total; kb += kTileThreads) {
+        const int cb = flat_col<TW>(kb, s_pre);
+        const uint32_t k = kb + lane;
+        if (k >= total) continue;
+        const int c = col_from<TW>(k, cb, s_pre);
+        const uint32_t i = s_rb[c] + (k - s_pre[c]);
+        // issue every load before testing the flags
         const uint32_t u = L.u(ICE_IDX_UTILITY, i);
-        if (u & (kDisabled | kEscaped)) continue;
         G2PState s;
         load_g2p_state(L, i, u, s);
         const float th = L(ICE_IDX_THICKNESS, i);
+        const float dpt = dp_threshold(P, L, i);
+        if (u & (kDisabled | kEscaped)) continue;
         const int rx = s.cx - x0 + 2, ry = s.cy - y0 + 2;
         if (rx < 1 || rx > RW - 2 || ry < 1 || ry > RH - 2) {
             err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
@@ -240,7 +282,7 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiled(DevParams P, StepArg
                 C.a10 += f * (nv.y * dx);
                 C.a11 += f * (nv.y * dy);
             }
-        g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err);
+        g2p_finish(P, s, pvx, pvy, C, th, dpt, err);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
         if (s.cell_updated && !(s.u & kDisabled)) {
             if (s.u & kMoved) {

@@ -183,7 +183,9 @@ int ice_get_timings(IceHandle *h, double out[8]);
 int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
 /* Tuning switches of the sorted path (no reference counterpart):
  * "p2g_point_lanes" = 1 maps one lane per point in the tiled P2G instead of
- * one lane per cell (ignored in the deterministic mode). */
+ * one lane per cell (ignored in the deterministic mode); "band" (1..4096,
+ * default 4) is the number of tile columns per band of the workgroup-to-tile
+ * order. */
 int ice_set_option(IceHandle *h, const char *name, int value);
 
 /* ---- introspection */
