@@ -59,6 +59,7 @@ class IceInfo(C.Structure):
         ("grid_x", C.c_int32), ("grid_y", C.c_int32), ("device", C.c_int32),
         ("sorted", C.c_int32), ("steps_since_sort", C.c_int32), ("n_sorts", C.c_int32),
         ("flags", C.c_uint32), ("tile_w", C.c_int32), ("tile_h", C.c_int32), ("n_offtile", C.c_uint32),
+        ("n_domains", C.c_int32), ("n_tiles", C.c_int32),
     ]
 
 
@@ -96,6 +97,12 @@ SIGNATURES = {
     "ice_set_option": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "ice_get_info": (C.c_int, [_P, C.POINTER(IceInfo)]),
     "ice_copy_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "ice_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "ice_create_batch": (C.c_int, [C.POINTER(IceParams), C.c_int, C.c_int, _P, _P, _P, C.POINTER(_P)]),
+    "ice_upload_domain": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.c_uint32, C.c_int, _P, _P, _P]),
+    "ice_download_domain": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.c_int, _P, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_int32)]),
+    "ice_get_domain_info": (C.c_int, [_P, C.c_int, C.POINTER(IceInfo)]),
 }
 
 _lib = None

@@ -17,8 +17,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libicempm.so")
 
 SOURCES = ["icempm_abi.hip"]
-HEADERS = ["device_math.hpp", "point_ops.hpp", "kernels_naive.hpp", "kernels_tiled.hpp",
-           "kernels_sort.hpp", "kernels_p2g_cells.hpp"]
+HEADERS = ["device_math.hpp", "point_ops.hpp", "kernels_naive.hpp", "kernels_sort.hpp", "tile_common.hpp",
+           "kernels_p2g.hpp", "kernels_g2p.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

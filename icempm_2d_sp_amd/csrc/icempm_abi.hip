@@ -1,24 +1,30 @@
 // icempm_abi.hip — the C-ABI of include/icempm.h on one MI355X (gfx950).
 //
-// One IceHandle = one GPU_Partition of the reference (gpu_partition.h:63-134):
-// it owns the device SoA (two buffers, for the sort), the grid lines, the
-// status bytes, the stream and the timing events.  Nothing here throws across
-// the ABI; failures return ICE_E_* and leave a message in ice_last_error.
+// One IceHandle stands for the reference's GPU_Partition (gpu_partition.h:63-134)
+// and owns the device SoA (two buffers, for the sort), the grid lines, the
+// status bytes, the stream and the timing events.  A handle holds one domain
+// (ice_create) or several independent domains (ice_create_batch) that share
+// every buffer and every launch: one launch per phase covers all tiles of all
+// domains.  Nothing here throws across the ABI; failures return ICE_E_* and
+// leave a message in ice_last_error.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
-#include <algorithm>
 
 #include "../../include/icempm.h"
 #include "device_math.hpp"
 #include "point_ops.hpp"
 #include "kernels_naive.hpp"
-#include "kernels_tiled.hpp"
 #include "kernels_sort.hpp"
-#include "kernels_p2g_cells.hpp"
+#include "tile_common.hpp"
+#include "kernels_p2g.hpp"
+#include "kernels_g2p.hpp"
 
 using namespace ice;
 
@@ -36,6 +42,17 @@ struct TimingSet {
     hipEvent_t e[5];
 };
 
+struct Domain {
+    int gx = 0, gy = 0;
+    size_t N = 0;
+    uint32_t capacity = 0, n = 0;
+    uint32_t pt_base = 0;          // also the base of the domain's off-tile list
+    size_t node_base = 0, cs_base = 0;
+    bool has_current = false;
+    bool sorted = false;
+    int steps_since_sort = 0, n_sorts = 0;
+};
+
 }  // namespace
 
 struct IceHandle {
@@ -44,30 +61,36 @@ struct IceHandle {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    bool batch = false;
+    std::vector<Domain> dom;
 
-    // points
+    // points (all domains)
     float *pts = nullptr, *pts_alt = nullptr;
     size_t pitch = 0;
-    uint32_t capacity = 0, n = 0;
+    uint32_t total_capacity = 0;
     int nlines = ICE_N_LINES;
-    // grid
+    // grid (all domains)
     float *grid = nullptr;
-    size_t gpitch = 0;
+    size_t gpitch = 0, total_nodes = 0;
     uint8_t *status = nullptr;
-    bool has_current = false;
-    Counters *cnt = nullptr;
+    Counters *cnt = nullptr;       // [ndom + 1]; cnt[ndom].step is the device step counter
+    Counters *host_cnt = nullptr;  // pinned copy
     uint32_t *offtile = nullptr;
-    // sort scratch
-    uint32_t *s_count = nullptr, *s_cs = nullptr, *s_key = nullptr, *s_slot = nullptr, *s_dflag = nullptr,
-             *s_drank = nullptr, *s_order = nullptr, *s_bsum = nullptr;
+    uint32_t *cs = nullptr;        // cell starts of every domain
+    size_t cs_total = 0;
+    DomainDesc *d_desc = nullptr;
+    TileRef *d_tiles = nullptr;
+    size_t tiles_cap = 0;
+    int ntiles = 0;
+    bool tables_dirty = true;
+    // sort scratch, sized for the largest domain
+    uint32_t *s_count = nullptr, *s_key = nullptr, *s_slot = nullptr, *s_dflag = nullptr, *s_drank = nullptr,
+             *s_order = nullptr, *s_bsum = nullptr;
     size_t s_bsum_len = 0;
-    bool sorted = false;
-    int steps_since_sort = 0, n_sorts = 0;
     // tiles
-    int tile_w = 32, tile_h = 32;
-    int p2g_point_lanes = 0;       // 1: one lane per point in the tiled P2G (the default is one per cell)
-    int band = 4;                  // tile columns per band of the workgroup -> tile order
+    int tile_w = 32, tile_h = 32, band = 4;
     FixScale fix{};
+    float max_thickness = 1.f;
     // phase state
     double pending_t = 0.0;
     double last_t = 0.0;
@@ -79,13 +102,26 @@ struct IceHandle {
     // graph
     hipGraphExec_t graph_exec = nullptr;
     int graph_chunk = 0;
-    uint32_t *host_cnt = nullptr;   // pinned copy of Counters
 
-    size_t N() const { return (size_t)prm.GridXTotal * prm.GridYTotal; }
+    int ndom() const { return (int)dom.size(); }
     float *gline(int l) const { return grid + (size_t)l * gpitch; }
     Lines lines() const { return Lines{pts, pitch}; }
+    Lines dlines(int d) const { return Lines{pts + dom[d].pt_base, pitch}; }
+    float *dgline(int l, int d) const { return gline(l) + dom[d].node_base; }
+    Counters *dcnt(int d) const { return cnt + d; }
+    Counters *step_cnt() const { return cnt + dom.size(); }
     bool naive() const { return (prm.flags & ICE_FLAG_NAIVE) != 0; }
     bool det() const { return (prm.flags & ICE_FLAG_DETERMINISTIC) != 0; }
+    uint32_t max_capacity() const {
+        uint32_t m = 1;
+        for (auto &d : dom) m = std::max(m, d.capacity);
+        return m;
+    }
+    size_t max_nodes() const {
+        size_t m = 1;
+        for (auto &d : dom) m = std::max(m, d.N);
+        return m;
+    }
 };
 
 #define HIPCHK(h, expr)                                                                 \
@@ -121,7 +157,7 @@ void derive_params(IceParams &p) {
     p.vmax = 0.25 * p.cellsize / p.InitialTimeStep;
 }
 
-DevParams make_dev_params(const IceParams &p, bool has_current) {
+DevParams make_dev_params(const IceParams &p) {
     DevParams d{};
     d.gx = p.GridXTotal;
     d.gy = p.GridYTotal;
@@ -157,7 +193,7 @@ DevParams make_dev_params(const IceParams &p, bool has_current) {
     d.DP_threshold_p = (float)p.DP_threshold_p;
     d.ridge = (float)(p.RidgeFormationCoeff * 9.8 * p.IceDensity);
     d.has_cohesion = (p.flags & ICE_FLAG_COHESION_LINE) ? 1 : 0;
-    d.has_current = has_current ? 1 : 0;
+    d.has_current = 0;   // per domain (DomainDesc)
     d.update_every = p.UpdateEveryNthStep;
     d.dt_d = p.InitialTimeStep;
     return d;
@@ -167,10 +203,10 @@ DevParams make_dev_params(const IceParams &p, bool has_current) {
 // 2^62 for up to 4096 point-equivalents of the heaviest point at |v| = vmax,
 // plus a stress term at the compressive strength.
 FixScale make_fix_scale(const IceParams &p, float max_thickness) {
-    const double mp = p.ParticleMass * std::max(1.0f, max_thickness);
-    const double m_bound = 4096.0 * mp;
+    const double th = std::max(1.0f, max_thickness);
+    const double m_bound = 4096.0 * p.ParticleMass * th;
     const double stress = p.dt_vol_Dpinv * 4.0 * std::max(p.IceCompressiveStrength, p.kappa) * p.cellsize;
-    const double p_bound = m_bound * p.vmax * 2.0 + 4096.0 * stress * std::max(1.0f, max_thickness);
+    const double p_bound = m_bound * p.vmax * 2.0 + 4096.0 * stress * th;
     FixScale f;
     const double lim = std::ldexp(1.0, 62);
     const int em = (int)std::floor(std::log2(lim / m_bound));
@@ -202,22 +238,18 @@ int launch_fill_u32(IceHandle *h, uint32_t *a, size_t n, uint32_t v) {
 
 // ---- tile dispatch ----------------------------------------------------------
 
-using P2GFn = void (*)(DevParams, Lines, const uint32_t *, float *, float *, float *, Counters *, int, int, FixScale);
-using P2GCellFn = void (*)(DevParams, Lines, const uint32_t *, float *, float *, float *, Counters *, int, int,
-                           FixScale, const uint32_t *, int);
-using G2PFn = void (*)(DevParams, StepArg, Lines, const uint32_t *, GridIn, float *, float *, Counters *, uint32_t *,
-                       int, int, int);
+using P2GFn = void (*)(DevParams, const DomainDesc *, const TileRef *, int, Lines, const uint32_t *, GridLines,
+                       Counters *, uint32_t *, FixScale);
+using G2PFn = void (*)(DevParams, StepArg, const DomainDesc *, const TileRef *, int, Lines, const uint32_t *,
+                       GridLines, Counters *, uint32_t *);
 
 struct TileKernels {
     int w, h;
-    P2GFn p2g, p2g_det;            // one lane per point
-    P2GCellFn p2gc, p2gc_det;      // one lane per cell
+    P2GFn p2g, p2g_det;
     G2PFn g2p;
 };
 
-#define TK(W, H)                                                                                      \
-    {W, H, k_p2g_tiled<W, H, false>, k_p2g_tiled<W, H, true>, k_p2g_cells<W, H, false>,             \
-     k_p2g_cells<W, H, true>, k_g2p_tiled<W, H>}
+#define TK(W, H) {W, H, k_p2g_tiles<W, H, false>, k_p2g_tiles<W, H, true>, k_g2p_tiles<W, H>}
 const TileKernels kTiles[] = {TK(8, 8),   TK(8, 16),  TK(16, 8),  TK(16, 16), TK(16, 32),
                               TK(32, 16), TK(32, 32), TK(32, 64), TK(64, 32), TK(64, 64)};
 #undef TK
@@ -226,6 +258,68 @@ const TileKernels *find_tile(int w, int h) {
     for (const auto &t : kTiles)
         if (t.w == w && t.h == h) return &t;
     return nullptr;
+}
+
+GridLines grid_lines(const IceHandle *h) {
+    GridLines G;
+    G.m = h->gline(G_M);
+    G.px = h->gline(G_PX);
+    G.py = h->gline(G_PY);
+    G.cur_x = h->gline(G_CX);
+    G.cur_y = h->gline(G_CY);
+    G.fx = h->gline(G_FX);
+    G.fy = h->gline(G_FY);
+    G.wind_x = h->gline(G_WX);
+    G.wind_y = h->gline(G_WY);
+    G.status = h->status;
+    return G;
+}
+
+void drop_graph(IceHandle *h) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    h->graph_chunk = 0;
+}
+
+// Domain and tile tables: domains by decreasing node count, inside a domain
+// bands of `band` tile columns, rows outer, columns inner.
+int upload_tables(IceHandle *h) {
+    if (!h->tables_dirty) return ICE_OK;
+    const int nd = h->ndom();
+    std::vector<DomainDesc> desc(nd);
+    for (int d = 0; d < nd; d++) {
+        const Domain &D = h->dom[d];
+        desc[d] = DomainDesc{D.gx, D.gy, D.has_current ? 1 : 0, 0, D.pt_base, D.pt_base, D.node_base, D.cs_base};
+    }
+    std::vector<int> order(nd);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h->dom[a].N > h->dom[b].N; });
+    std::vector<TileRef> tiles;
+    for (int d : order) {
+        const int ntx = (h->dom[d].gx + h->tile_w - 1) / h->tile_w;
+        const int nty = (h->dom[d].gy + h->tile_h - 1) / h->tile_h;
+        for (int b0 = 0; b0 < ntx; b0 += h->band) {
+            const int wb = std::min(h->band, ntx - b0);
+            for (int ty = 0; ty < nty; ty++)
+                for (int k = 0; k < wb; k++) tiles.push_back(TileRef{(uint16_t)d, (uint16_t)(b0 + k), (uint16_t)ty, 0});
+        }
+    }
+    if (tiles.size() > h->tiles_cap) {
+        if (h->d_tiles) (void)hipFree(h->d_tiles);
+        h->d_tiles = nullptr;
+        HIPCHK(h, hipMalloc(&h->d_tiles, tiles.size() * sizeof(TileRef)));
+        h->tiles_cap = tiles.size();
+    }
+    if (!h->d_desc) HIPCHK(h, hipMalloc(&h->d_desc, std::max(1, nd) * sizeof(DomainDesc)));
+    HIPCHK(h, hipMemcpyAsync(h->d_desc, desc.data(), nd * sizeof(DomainDesc), hipMemcpyHostToDevice, h->stream));
+    if (!tiles.empty())
+        HIPCHK(h, hipMemcpyAsync(h->d_tiles, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
+                                 h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->ntiles = (int)tiles.size();
+    h->tables_dirty = false;
+    drop_graph(h);
+    return ICE_OK;
 }
 
 int scan_u32(IceHandle *h, const uint32_t *in, size_t n, uint32_t *out) {
@@ -243,68 +337,61 @@ int scan_u32(IceHandle *h, const uint32_t *in, size_t n, uint32_t *out) {
     return ICE_OK;
 }
 
-void drop_graph(IceHandle *h);
-
-// Sort points into cell order; keep disabled points behind (drop = false) or
-// drop them (drop = true).  Leaves cs valid and the off-tile list empty.
-int device_sort(IceHandle *h, bool drop, uint32_t *perm_dev) {
-    const uint32_t n = h->n;
-    const uint32_t NC = (uint32_t)h->N();
+// Sort domain d's points into cell order; keep disabled points behind (drop =
+// false) or drop them (drop = true).  Leaves the domain's cs valid and its
+// off-tile list empty.
+int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
+    Domain &D = h->dom[d];
+    const uint32_t n = D.n;
+    const uint32_t NC = (uint32_t)D.N;
+    uint32_t *cs = h->cs + D.cs_base;
     int rc;
     if ((rc = launch_fill_u32(h, h->s_count, (size_t)NC + 1, 0u))) return rc;
     if (n) {
-        hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->lines(), n,
-                           h->prm.GridXTotal, h->prm.GridYTotal, NC, h->s_count, h->s_key, h->s_slot, h->s_dflag,
-                           h->cnt);
+        hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->dlines(d), n, D.gx, D.gy,
+                           NC, h->s_count, h->s_key, h->s_slot, h->s_dflag, h->dcnt(d));
         LAUNCHCHK(h, "k_sort_keys");
     }
-    if ((rc = scan_u32(h, h->s_count, NC, h->s_cs))) return rc;
+    if ((rc = scan_u32(h, h->s_count, NC, cs))) return rc;
     if ((rc = scan_u32(h, h->s_dflag, n, h->s_drank))) return rc;
     if (n) {
         hipLaunchKernelGGL(k_sort_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, NC, h->s_key,
-                           h->s_slot, h->s_cs, h->s_drank, h->s_order);
+                           h->s_slot, cs, h->s_drank, h->s_order);
         LAUNCHCHK(h, "k_sort_scatter");
     }
-    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(NC, 256)), dim3(256), 0, h->stream, NC, h->s_cs, h->s_order);
+    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(NC, 256)), dim3(256), 0, h->stream, NC, cs, h->s_order);
     LAUNCHCHK(h, "k_sort_cells");
     uint32_t n_active = 0;
-    HIPCHK(h, hipMemcpyAsync(&n_active, h->s_cs + NC, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&n_active, cs + NC, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     const uint32_t n_out = drop ? n_active : n;
     if (n_out) {
-        Lines dst{h->pts_alt, h->pitch};
-        hipLaunchKernelGGL(k_sort_gather, dim3(blocks_for(n_out, 256)), dim3(256), 0, h->stream, h->lines(), dst,
+        Lines dst{h->pts_alt + D.pt_base, h->pitch};
+        hipLaunchKernelGGL(k_sort_gather, dim3(blocks_for(n_out, 256)), dim3(256), 0, h->stream, h->dlines(d), dst,
                            h->nlines, n_out, h->s_order, perm_dev);
         LAUNCHCHK(h, "k_sort_gather");
+        if (h->ndom() == 1) {
+            std::swap(h->pts, h->pts_alt);
+        } else {   // other domains live in the same buffers: copy the sorted range back
+            HIPCHK(h, hipMemcpy2DAsync(h->pts + D.pt_base, h->pitch * sizeof(float), h->pts_alt + D.pt_base,
+                                       h->pitch * sizeof(float), n_out * sizeof(float), h->nlines,
+                                       hipMemcpyDeviceToDevice, h->stream));
+        }
     }
-    std::swap(h->pts, h->pts_alt);
-    h->n = n_out;
     drop_graph(h);   // a captured graph holds the old point buffer
+    D.n = n_out;
+    Counters *c = h->dcnt(d);
     if (drop) {
         const int32_t zero = 0;
-        HIPCHK(h, hipMemcpyAsync(&h->cnt->disabled, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, hipMemcpyAsync(&c->disabled, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
     }
     const uint32_t zero_u = 0;
-    HIPCHK(h, hipMemcpyAsync(&h->cnt->n_esc, &zero_u, sizeof(zero_u), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&c->n_esc, &zero_u, sizeof(zero_u), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->sorted = true;
-    h->steps_since_sort = 0;
-    h->n_sorts++;
+    D.sorted = true;
+    D.steps_since_sort = 0;
+    D.n_sorts++;
     return ICE_OK;
-}
-
-GridIn grid_in(const IceHandle *h) {
-    GridIn G;
-    G.m = h->gline(G_M);
-    G.px = h->gline(G_PX);
-    G.py = h->gline(G_PY);
-    G.cur_x = h->gline(G_CX);
-    G.cur_y = h->gline(G_CY);
-    G.wind_x = h->gline(G_WX);
-    G.wind_y = h->gline(G_WY);
-    G.status = h->status;
-    return G;
 }
 
 TimingSet *cur_timing(IceHandle *h) {
@@ -323,45 +410,57 @@ void rec(IceHandle *h, int k) {
     if (t) (void)hipEventRecord(t->e[k], h->stream);
 }
 
-unsigned offtile_blocks(const IceHandle *h) { return std::min<unsigned>(512, blocks_for(std::max<uint32_t>(h->n, 1), 256)); }
+unsigned offtile_blocks(const IceHandle *h) {
+    return std::min<unsigned>(64, blocks_for(h->max_capacity(), 4096));
+}
+
+DevParams dom_params(const IceHandle *h, int d) {
+    DevParams P = h->dp;
+    P.gx = h->dom[d].gx;
+    P.gy = h->dom[d].gy;
+    P.has_current = h->dom[d].has_current ? 1 : 0;
+    return P;
+}
 
 // Phases of one substep.  Event slots: 0 start, 1 after the main P2G kernel,
 // 2 after P2G, 3 before the main G2P kernel, 4 after G2P.
 int phase_p2g(IceHandle *h) {
     if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
-    const DevParams &P = h->dp;
     if (h->naive()) {
-        int rc = launch_fill(h, h->grid, 3 * h->gpitch);   // reset_grid (gpu_partition.cpp:300-308)
-        if (rc) return rc;
-        if (h->n) {
-            hipLaunchKernelGGL(k_p2g_naive, dim3(blocks_for(h->n, 256)), dim3(256), 0, h->stream, P, h->lines(), h->n,
-                               h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt);
-            LAUNCHCHK(h, "k_p2g_naive");
+        for (int d = 0; d < h->ndom(); d++) {
+            for (int l = G_M; l <= G_PY; l++) {   // reset_grid (gpu_partition.cpp:300-308)
+                int rc = launch_fill(h, h->dgline(l, d), h->dom[d].N);
+                if (rc) return rc;
+            }
+            if (h->dom[d].n) {
+                hipLaunchKernelGGL(k_p2g_naive, dim3(blocks_for(h->dom[d].n, 256)), dim3(256), 0, h->stream,
+                                   dom_params(h, d), h->dlines(d), h->dom[d].n, h->dgline(G_M, d), h->dgline(G_PX, d),
+                                   h->dgline(G_PY, d), h->dcnt(d));
+                LAUNCHCHK(h, "k_p2g_naive");
+            }
         }
         rec(h, 1);
         rec(h, 2);
         return ICE_OK;
     }
-    if (!h->sorted) {
-        int rc = device_sort(h, false, nullptr);
-        if (rc) return rc;
-    }
+    for (int d = 0; d < h->ndom(); d++)
+        if (!h->dom[d].sorted) {
+            int rc = device_sort(h, d, false, nullptr);
+            if (rc) return rc;
+        }
+    int rc = upload_tables(h);
+    if (rc) return rc;
     const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
-    const int ntx = (P.gx + h->tile_w - 1) / h->tile_w, nty = (P.gy + h->tile_h - 1) / h->tile_h;
-    if (h->p2g_point_lanes && !h->det()) {
-        hipLaunchKernelGGL(tk->p2g, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, h->lines(), h->s_cs,
-                           h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt, ntx, nty, h->fix);
-    } else {
-        P2GCellFn fn = h->det() ? tk->p2gc_det : tk->p2gc;
-        hipLaunchKernelGGL(fn, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, h->lines(), h->s_cs,
-                           h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt, ntx, nty, h->fix,
-                           (const uint32_t *)h->offtile, h->band);
+    const GridLines G = grid_lines(h);
+    if (h->ntiles) {
+        hipLaunchKernelGGL(h->det() ? tk->p2g_det : tk->p2g, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp,
+                           h->d_desc, h->d_tiles, h->ntiles, h->lines(), h->cs, G, h->cnt, h->offtile, h->fix);
+        LAUNCHCHK(h, "k_p2g_tiles");
     }
-    LAUNCHCHK(h, "k_p2g_tiled");
     rec(h, 1);
     if (!h->det()) {   // the deterministic tile kernel adds the off-tile points itself
-        hipLaunchKernelGGL(k_p2g_offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, P, h->lines(),
-                           h->offtile, h->gline(G_M), h->gline(G_PX), h->gline(G_PY), h->cnt);
+        hipLaunchKernelGGL(k_p2g_offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp,
+                           h->d_desc, h->lines(), G, h->cnt, (const uint32_t *)h->offtile);
         LAUNCHCHK(h, "k_p2g_offtile");
     }
     rec(h, 2);
@@ -369,40 +468,44 @@ int phase_p2g(IceHandle *h) {
 }
 
 int phase_update(IceHandle *h, const StepArg &S) {
-    if (h->naive()) {
-        const size_t N = h->N();
-        hipLaunchKernelGGL(k_update_naive, dim3(blocks_for(N, 256)), dim3(256), 0, h->stream, h->dp, S, h->gline(G_M),
-                           h->gline(G_PX), h->gline(G_PY), h->status, h->gline(G_CX), h->gline(G_CY), h->gline(G_WX),
-                           h->gline(G_WY), h->gline(G_FX), h->gline(G_FY), h->cnt);
+    if (!h->naive()) return ICE_OK;   // fused into the tile G2P
+    for (int d = 0; d < h->ndom(); d++) {
+        const size_t N = h->dom[d].N;
+        hipLaunchKernelGGL(k_update_naive, dim3(blocks_for(N, 256)), dim3(256), 0, h->stream, dom_params(h, d), S,
+                           h->dgline(G_M, d), h->dgline(G_PX, d), h->dgline(G_PY, d), h->status + h->dom[d].node_base,
+                           h->dgline(G_CX, d), h->dgline(G_CY, d), h->dgline(G_WX, d), h->dgline(G_WY, d),
+                           h->dgline(G_FX, d), h->dgline(G_FY, d), h->dcnt(d));
         LAUNCHCHK(h, "k_update_naive");
     }
     return ICE_OK;
 }
 
 int phase_g2p(IceHandle *h, const StepArg &S) {
-    const DevParams &P = h->dp;
     if (h->naive()) {
         rec(h, 3);
-        if (h->n) {
-            hipLaunchKernelGGL(k_g2p_naive, dim3(blocks_for(h->n, 256)), dim3(256), 0, h->stream, P, S, h->lines(),
-                               h->n, (const float *)h->gline(G_PX), (const float *)h->gline(G_PY), h->cnt);
-            LAUNCHCHK(h, "k_g2p_naive");
-        }
+        for (int d = 0; d < h->ndom(); d++)
+            if (h->dom[d].n) {
+                hipLaunchKernelGGL(k_g2p_naive, dim3(blocks_for(h->dom[d].n, 256)), dim3(256), 0, h->stream,
+                                   dom_params(h, d), S, h->dlines(d), h->dom[d].n, (const float *)h->dgline(G_PX, d),
+                                   (const float *)h->dgline(G_PY, d), h->dcnt(d));
+                LAUNCHCHK(h, "k_g2p_naive");
+            }
         rec(h, 4);
         return ICE_OK;
     }
-    const GridIn G = grid_in(h);
-    hipLaunchKernelGGL(k_g2p_offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, P, S, h->lines(), h->offtile, G,
-                       h->cnt);
+    const GridLines G = grid_lines(h);
+    hipLaunchKernelGGL(k_g2p_offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S, h->d_desc,
+                       h->lines(), G, h->cnt, (const uint32_t *)h->offtile);
     LAUNCHCHK(h, "k_g2p_offtile");
     rec(h, 3);
-    const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
-    const int ntx = (P.gx + h->tile_w - 1) / h->tile_w, nty = (P.gy + h->tile_h - 1) / h->tile_h;
-    hipLaunchKernelGGL(tk->g2p, dim3(ntx * nty), dim3(kTileThreads), 0, h->stream, P, S, h->lines(), h->s_cs, G,
-                       h->gline(G_FX), h->gline(G_FY), h->cnt, h->offtile, ntx, nty, h->band);
-    LAUNCHCHK(h, "k_g2p_tiled");
+    if (h->ntiles) {
+        const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
+        hipLaunchKernelGGL(tk->g2p, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, S, h->d_desc, h->d_tiles,
+                           h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt, h->offtile);
+        LAUNCHCHK(h, "k_g2p_tiles");
+    }
     rec(h, 4);
-    h->steps_since_sort++;
+    for (auto &D : h->dom) D.steps_since_sort++;
     return ICE_OK;
 }
 
@@ -420,17 +523,11 @@ int substep_args(IceHandle *h, const StepArg &S) {
     return ICE_OK;
 }
 
-int read_counters(IceHandle *h, Counters &c) {
-    HIPCHK(h, hipMemcpyAsync(h->host_cnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+int read_counters(IceHandle *h) {
+    HIPCHK(h, hipMemcpyAsync(h->host_cnt, h->cnt, sizeof(Counters) * (h->dom.size() + 1), hipMemcpyDeviceToHost,
+                             h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    std::memcpy(&c, h->host_cnt, sizeof(Counters));
     return ICE_OK;
-}
-
-void drop_graph(IceHandle *h) {
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    h->graph_exec = nullptr;
-    h->graph_chunk = 0;
 }
 
 int build_graph(IceHandle *h, int chunk) {
@@ -441,21 +538,258 @@ int build_graph(IceHandle *h, int chunk) {
     h->timing = false;
     int rc = ICE_OK;
     for (int k = 0; k < chunk && rc == ICE_OK; k++) {
-        StepArg S{0.0, 0, &h->cnt->step, k};
+        StepArg S{0.0, 0, &h->step_cnt()->step, k};
         rc = substep_args(h, S);
     }
-    if (rc == ICE_OK) {
-        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, h->stream, h->cnt, chunk);
-    }
+    if (rc == ICE_OK) hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, h->stream, h->step_cnt(), chunk);
     h->timing = was_timing;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
     if (e != hipSuccess) { h->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e); return ICE_E_HIP; }
     e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) { h->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e); return ICE_E_HIP; }
     h->graph_chunk = chunk;
     return ICE_OK;
+}
+
+bool grid_ok(int gx, int gy, std::string &why) {
+    if (gx < 5 || gy < 5 || gx > 65535 || gy > 65535) {
+        why = "grid size out of range (5..65535 nodes per axis; the cell word holds 16 bits per axis)";
+        return false;
+    }
+    if ((uint64_t)gx * gy >= 0xffffffffull) { why = "grid too large"; return false; }
+    return true;
+}
+
+// Allocate everything that depends only on the domains' grids.
+int alloc_grid(IceHandle *h) {
+    size_t nb = 0, cb = 0;
+    for (auto &D : h->dom) {
+        D.N = (size_t)D.gx * D.gy;
+        D.node_base = nb;
+        nb = round_up(nb + D.N, 64);
+        D.cs_base = cb;
+        cb = round_up(cb + D.N + 2, 64);
+    }
+    h->total_nodes = std::max<size_t>(nb, 64);
+    h->gpitch = h->total_nodes;
+    h->cs_total = std::max<size_t>(cb, 64);
+    const size_t maxN = h->max_nodes();
+    HIPCHK(h, hipMalloc(&h->grid, sizeof(float) * h->gpitch * kGridLines));
+    HIPCHK(h, hipMalloc(&h->status, round_up(h->total_nodes, 256)));
+    HIPCHK(h, hipMalloc(&h->cnt, sizeof(Counters) * (h->dom.size() + 1)));
+    HIPCHK(h, hipHostMalloc(&h->host_cnt, sizeof(Counters) * (h->dom.size() + 1)));
+    HIPCHK(h, hipMalloc(&h->s_count, sizeof(uint32_t) * (maxN + 1)));
+    HIPCHK(h, hipMalloc(&h->cs, sizeof(uint32_t) * h->cs_total));
+    HIPCHK(h, hipMemsetAsync(h->grid, 0, sizeof(float) * h->gpitch * kGridLines, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->status, ICE_GRID_STATUS_WATER, round_up(h->total_nodes, 256), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->cnt, 0, sizeof(Counters) * (h->dom.size() + 1), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->cs, 0, sizeof(uint32_t) * h->cs_total, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::memset(h->host_cnt, 0, sizeof(Counters) * (h->dom.size() + 1));
+    return ICE_OK;
+}
+
+// (Re)allocate the point buffers and the scratch that depends on capacity.
+int alloc_points(IceHandle *h) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graph(h);
+    for (void **p : {(void **)&h->pts, (void **)&h->pts_alt, (void **)&h->offtile, (void **)&h->s_key,
+                     (void **)&h->s_slot, (void **)&h->s_dflag, (void **)&h->s_drank, (void **)&h->s_order,
+                     (void **)&h->s_bsum}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    size_t base = 0;
+    for (auto &D : h->dom) {
+        D.pt_base = (uint32_t)base;
+        D.n = 0;
+        D.sorted = false;
+        base = round_up(base + std::max<uint32_t>(D.capacity, 1), 64);
+    }
+    if (base >= 0xffffffffull) { h->err = "total point capacity exceeds 2^32"; return ICE_E_CAPACITY; }
+    h->total_capacity = (uint32_t)base;
+    h->pitch = base;
+    const size_t bytes = sizeof(float) * h->pitch * ICE_N_LINES_MAX;
+    const size_t mc = h->max_capacity();
+    HIPCHK(h, hipMalloc(&h->pts, bytes));
+    HIPCHK(h, hipMalloc(&h->pts_alt, bytes));
+    HIPCHK(h, hipMemsetAsync(h->pts, 0, bytes, h->stream));
+    HIPCHK(h, hipMalloc(&h->offtile, sizeof(uint32_t) * h->total_capacity));
+    HIPCHK(h, hipMalloc(&h->s_key, sizeof(uint32_t) * mc));
+    HIPCHK(h, hipMalloc(&h->s_slot, sizeof(uint32_t) * mc));
+    HIPCHK(h, hipMalloc(&h->s_dflag, sizeof(uint32_t) * mc));
+    HIPCHK(h, hipMalloc(&h->s_drank, sizeof(uint32_t) * (mc + 1)));
+    HIPCHK(h, hipMalloc(&h->s_order, sizeof(uint32_t) * mc));
+    h->s_bsum_len = std::max<size_t>(mc, h->max_nodes()) / kScanTile + 8;
+    HIPCHK(h, hipMalloc(&h->s_bsum, sizeof(uint32_t) * h->s_bsum_len));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->tables_dirty = true;
+    return ICE_OK;
+}
+
+int create_common(const IceParams *p, int device, std::vector<Domain> doms, bool batch, IceHandle **out) {
+    if (!p || !out) { g_create_error = "null argument"; return ICE_E_ARG; }
+    *out = nullptr;
+    for (auto &D : doms)
+        if (!grid_ok(D.gx, D.gy, g_create_error)) return ICE_E_ARG;
+    if (doms.empty() || doms.size() > 65535) { g_create_error = "1..65535 domains"; return ICE_E_ARG; }
+    if (!(p->InitialTimeStep > 0) || !(p->cellsize > 0)) { g_create_error = "dt and cellsize must be > 0"; return ICE_E_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return ICE_E_DEVICE; }
+    if (device < 0 || device >= ndev) { g_create_error = "device index out of range"; return ICE_E_DEVICE; }
+    IceHandle *h = new IceHandle;
+    h->prm = *p;
+    derive_params(h->prm);
+    h->device = device;
+    h->batch = batch;
+    h->dom = std::move(doms);
+    auto fail = [&](int code) {
+        g_create_error = h->err;
+        ice_destroy(h);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) { h->err = "hipSetDevice failed"; return fail(ICE_E_HIP); }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { h->err = "stream"; return fail(ICE_E_HIP); }
+    int rc = alloc_grid(h);
+    if (rc) return fail(rc);
+    h->dp = make_dev_params(h->prm);
+    h->fix = make_fix_scale(h->prm, 1.f);
+    *out = h;
+    return ICE_OK;
+}
+
+int check_domain(IceHandle *h, int d) {
+    if (d < 0 || d >= h->ndom()) { h->err = "domain index out of range"; return ICE_E_ARG; }
+    return ICE_OK;
+}
+
+int upload_points_d(IceHandle *h, int d, const float *soa, size_t host_pitch, uint32_t n, int n_lines) {
+    if (check_domain(h, d)) return ICE_E_ARG;
+    if (!soa && n) return ICE_E_ARG;
+    if (!h->pts) { h->err = "allocate points first"; return ICE_E_STATE; }
+    Domain &D = h->dom[d];
+    if (n > D.capacity) { h->err = "more points than the allocated capacity"; return ICE_E_CAPACITY; }
+    if (n_lines < ICE_N_LINES || n_lines > ICE_N_LINES_MAX || host_pitch < n) { h->err = "bad layout"; return ICE_E_ARG; }
+    if ((h->prm.flags & ICE_FLAG_COHESION_LINE) && n_lines < ICE_N_LINES_MAX) {
+        h->err = "ICE_FLAG_COHESION_LINE needs 20 lines";
+        return ICE_E_ARG;
+    }
+    HIPCHK(h, hipSetDevice(h->device));
+    h->nlines = (h->prm.flags & ICE_FLAG_COHESION_LINE) ? ICE_N_LINES_MAX : ICE_N_LINES;
+    if (n)
+        HIPCHK(h, hipMemcpy2DAsync(h->pts + D.pt_base, h->pitch * sizeof(float), soa, host_pitch * sizeof(float),
+                                   n * sizeof(float), h->nlines, hipMemcpyHostToDevice, h->stream));
+    D.n = n;
+    // the partition's disabled count starts at 0 (gpu_implementation5.cpp:137, gpu_partition.cpp:101-103)
+    const int32_t zero = 0;
+    HIPCHK(h, hipMemcpyAsync(&h->dcnt(d)->disabled, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    if (n) {
+        const float *th = soa + (size_t)ICE_IDX_THICKNESS * host_pitch;
+        for (uint32_t i = 0; i < n; i++) h->max_thickness = std::max(h->max_thickness, th[i]);
+    }
+    h->fix = make_fix_scale(h->prm, h->max_thickness);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    D.sorted = false;
+    drop_graph(h);
+    if (!h->naive()) return device_sort(h, d, false, nullptr);
+    return ICE_OK;
+}
+
+int upload_grid_d(IceHandle *h, int d, const uint8_t *status, const float *normals) {
+    if (check_domain(h, d) || !status) return ICE_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    const Domain &D = h->dom[d];
+    HIPCHK(h, hipMemcpyAsync(h->status + D.node_base, status, D.N, hipMemcpyHostToDevice, h->stream));
+    if (normals) {
+        HIPCHK(h, hipMemcpyAsync(h->dgline(G_NX, d), normals, D.N * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->dgline(G_NY, d), normals + D.N, D.N * sizeof(float), hipMemcpyHostToDevice,
+                                 h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ICE_OK;
+}
+
+int upload_forcing_d(IceHandle *h, int d, const float *current, const float *wind) {
+    if (check_domain(h, d)) return ICE_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    Domain &D = h->dom[d];
+    int rc;
+    if (current) {
+        HIPCHK(h, hipMemcpyAsync(h->dgline(G_CX, d), current, D.N * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->dgline(G_CY, d), current + D.N, D.N * sizeof(float), hipMemcpyHostToDevice,
+                                 h->stream));
+        bool any = false;
+        for (size_t i = 0; i < 2 * D.N && !any; i++) any = current[i] != 0.f;
+        D.has_current = any;
+    } else {
+        if ((rc = launch_fill(h, h->dgline(G_CX, d), D.N))) return rc;
+        if ((rc = launch_fill(h, h->dgline(G_CY, d), D.N))) return rc;
+        D.has_current = false;
+    }
+    if (wind) {
+        HIPCHK(h, hipMemcpyAsync(h->dgline(G_WX, d), wind, D.N * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->dgline(G_WY, d), wind + D.N, D.N * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    } else {
+        if ((rc = launch_fill(h, h->dgline(G_WX, d), D.N))) return rc;
+        if ((rc = launch_fill(h, h->dgline(G_WY, d), D.N))) return rc;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->tables_dirty = true;
+    drop_graph(h);
+    return ICE_OK;
+}
+
+int download_d(IceHandle *h, int d, float *soa, size_t host_pitch, int n_lines, float *forces, uint32_t *error_code,
+               int32_t *disabled_count) {
+    if (check_domain(h, d)) return ICE_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    const Domain &D = h->dom[d];
+    if (soa) {
+        if (n_lines < ICE_N_LINES || n_lines > ICE_N_LINES_MAX || host_pitch < D.n) { h->err = "bad layout"; return ICE_E_ARG; }
+        const int nl = std::min(n_lines, h->nlines);
+        if (D.n)
+            HIPCHK(h, hipMemcpy2DAsync(soa, host_pitch * sizeof(float), h->pts + D.pt_base, h->pitch * sizeof(float),
+                                       D.n * sizeof(float), nl, hipMemcpyDeviceToHost, h->stream));
+    }
+    if (forces) {
+        HIPCHK(h, hipMemcpyAsync(forces, h->dgline(G_FX, d), D.N * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(forces + D.N, h->dgline(G_FY, d), D.N * sizeof(float), hipMemcpyDeviceToHost,
+                                 h->stream));
+    }
+    int rc = read_counters(h);
+    if (rc) return rc;
+    if (soa) {
+        uint32_t *u = reinterpret_cast<uint32_t *>(soa + (size_t)ICE_IDX_UTILITY * host_pitch);
+        for (uint32_t i = 0; i < D.n; i++) u[i] &= ~kInternalBits;
+    }
+    if (error_code) *error_code = h->host_cnt[d].err;
+    if (disabled_count) *disabled_count = h->host_cnt[d].disabled;
+    return ICE_OK;
+}
+
+void fill_info(IceHandle *h, int d, IceInfo *info) {
+    const Domain &D = h->dom[d];
+    info->n_points = D.n;
+    info->capacity = D.capacity;
+    info->point_pitch = h->pitch;
+    info->grid_pitch = h->gpitch;
+    info->grid_x = D.gx;
+    info->grid_y = D.gy;
+    info->device = h->device;
+    info->sorted = D.sorted ? 1 : 0;
+    info->steps_since_sort = D.steps_since_sort;
+    info->n_sorts = D.n_sorts;
+    info->flags = h->prm.flags;
+    info->tile_w = h->tile_w;
+    info->tile_h = h->tile_h;
+    info->n_offtile = h->host_cnt ? h->host_cnt[d].n_esc : 0;
+    info->n_domains = h->ndom();
+    info->n_tiles = h->ntiles;
 }
 
 }  // namespace
@@ -500,45 +834,34 @@ int ice_params_derive(IceParams *p) {
 const char *ice_last_error(const IceHandle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
 int ice_create(const IceParams *p, int device, IceHandle **out) {
-    if (!p || !out) { g_create_error = "null argument"; return ICE_E_ARG; }
-    *out = nullptr;
-    if (p->GridXTotal < 5 || p->GridYTotal < 5 || p->GridXTotal > 65535 || p->GridYTotal > 65535) {
-        g_create_error = "grid size out of range (5..65535 nodes per axis; the cell word holds 16 bits per axis)";
+    if (!p) { g_create_error = "null argument"; return ICE_E_ARG; }
+    Domain D;
+    D.gx = p->GridXTotal;
+    D.gy = p->GridYTotal;
+    return create_common(p, device, {D}, false, out);
+}
+
+int ice_create_batch(const IceParams *p, int device, int n_domains, const int32_t *grid_x, const int32_t *grid_y,
+                     const uint32_t *capacity, IceHandle **out) {
+    if (!p || !grid_x || !grid_y || !capacity || !out || n_domains <= 0) {
+        g_create_error = "bad arguments";
         return ICE_E_ARG;
     }
-    if ((uint64_t)p->GridXTotal * p->GridYTotal >= 0xffffffffull) { g_create_error = "grid too large"; return ICE_E_ARG; }
-    if (!(p->InitialTimeStep > 0) || !(p->cellsize > 0)) { g_create_error = "dt and cellsize must be > 0"; return ICE_E_ARG; }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return ICE_E_DEVICE; }
-    if (device < 0 || device >= ndev) { g_create_error = "device index out of range"; return ICE_E_DEVICE; }
-    IceHandle *h = new IceHandle;
-    h->prm = *p;
-    derive_params(h->prm);
-    h->device = device;
-    auto fail = [&](const std::string &m, int code) {
-        g_create_error = m;
-        ice_destroy(h);
-        return code;
-    };
-    if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice failed", ICE_E_HIP);
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream", ICE_E_HIP);
-    const size_t N = h->N();
-    h->gpitch = round_up(N, 64);
-    if (hipMalloc(&h->grid, sizeof(float) * h->gpitch * kGridLines) != hipSuccess) return fail("grid alloc", ICE_E_HIP);
-    if (hipMalloc(&h->status, round_up(N, 256)) != hipSuccess) return fail("status alloc", ICE_E_HIP);
-    if (hipMalloc(&h->cnt, sizeof(Counters)) != hipSuccess) return fail("counter alloc", ICE_E_HIP);
-    if (hipHostMalloc(&h->host_cnt, sizeof(Counters)) != hipSuccess) return fail("pinned alloc", ICE_E_HIP);
-    if (hipMalloc(&h->s_count, sizeof(uint32_t) * (N + 1)) != hipSuccess) return fail("sort alloc", ICE_E_HIP);
-    if (hipMalloc(&h->s_cs, sizeof(uint32_t) * (N + 2)) != hipSuccess) return fail("sort alloc", ICE_E_HIP);
-    if (hipMemsetAsync(h->grid, 0, sizeof(float) * h->gpitch * kGridLines, h->stream) != hipSuccess) return fail("memset", ICE_E_HIP);
-    if (hipMemsetAsync(h->status, ICE_GRID_STATUS_WATER, round_up(N, 256), h->stream) != hipSuccess) return fail("memset", ICE_E_HIP);
-    if (hipMemsetAsync(h->cnt, 0, sizeof(Counters), h->stream) != hipSuccess) return fail("memset", ICE_E_HIP);
-    if (hipMemsetAsync(h->s_cs, 0, sizeof(uint32_t) * (N + 2), h->stream) != hipSuccess) return fail("memset", ICE_E_HIP);
-    if (hipStreamSynchronize(h->stream) != hipSuccess) return fail("sync", ICE_E_HIP);
-    h->dp = make_dev_params(h->prm, false);
-    h->fix = make_fix_scale(h->prm, 1.f);
-    *out = h;
-    return ICE_OK;
+    std::vector<Domain> doms(n_domains);
+    for (int d = 0; d < n_domains; d++) {
+        doms[d].gx = grid_x[d];
+        doms[d].gy = grid_y[d];
+        doms[d].capacity = capacity[d];
+    }
+    int rc = create_common(p, device, std::move(doms), true, out);
+    if (rc) return rc;
+    rc = alloc_points(*out);
+    if (rc) {
+        g_create_error = (*out)->err;
+        ice_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
 }
 
 int ice_destroy(IceHandle *h) {
@@ -549,8 +872,9 @@ int ice_destroy(IceHandle *h) {
     for (auto &t : h->ev_pool)
         for (auto &e : t.e) (void)hipEventDestroy(e);
     for (void *p : {(void *)h->pts, (void *)h->pts_alt, (void *)h->grid, (void *)h->status, (void *)h->cnt,
-                    (void *)h->offtile, (void *)h->s_count, (void *)h->s_cs, (void *)h->s_key, (void *)h->s_slot,
-                    (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum})
+                    (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
+                    (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
+                    (void *)h->d_desc, (void *)h->d_tiles})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -560,120 +884,54 @@ int ice_destroy(IceHandle *h) {
 
 int ice_allocate(IceHandle *h, uint32_t cap) {
     if (!h) return ICE_E_ARG;
+    if (h->batch) { h->err = "capacities of a batch are fixed by ice_create_batch"; return ICE_E_STATE; }
     HIPCHK(h, hipSetDevice(h->device));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    drop_graph(h);
-    for (void **p : {(void **)&h->pts, (void **)&h->pts_alt, (void **)&h->offtile, (void **)&h->s_key,
-                     (void **)&h->s_slot, (void **)&h->s_dflag, (void **)&h->s_drank, (void **)&h->s_order,
-                     (void **)&h->s_bsum}) {
-        if (*p) (void)hipFree(*p);
-        *p = nullptr;
-    }
-    h->capacity = cap;
-    h->n = 0;
-    h->sorted = false;
-    const size_t c = std::max<uint32_t>(cap, 1);
-    h->pitch = round_up(c, 64);
-    const size_t bytes = sizeof(float) * h->pitch * ICE_N_LINES_MAX;
-    HIPCHK(h, hipMalloc(&h->pts, bytes));
-    HIPCHK(h, hipMalloc(&h->pts_alt, bytes));
-    HIPCHK(h, hipMemsetAsync(h->pts, 0, bytes, h->stream));
-    HIPCHK(h, hipMalloc(&h->offtile, sizeof(uint32_t) * c));
-    HIPCHK(h, hipMalloc(&h->s_key, sizeof(uint32_t) * c));
-    HIPCHK(h, hipMalloc(&h->s_slot, sizeof(uint32_t) * c));
-    HIPCHK(h, hipMalloc(&h->s_dflag, sizeof(uint32_t) * c));
-    HIPCHK(h, hipMalloc(&h->s_drank, sizeof(uint32_t) * (c + 1)));
-    HIPCHK(h, hipMalloc(&h->s_order, sizeof(uint32_t) * c));
-    h->s_bsum_len = std::max(c, h->N()) / kScanTile + 8;
-    HIPCHK(h, hipMalloc(&h->s_bsum, sizeof(uint32_t) * h->s_bsum_len));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    return ICE_OK;
+    h->dom[0].capacity = cap;
+    return alloc_points(h);
 }
 
 int ice_upload_points(IceHandle *h, const float *soa, size_t host_pitch, uint32_t n, int n_lines) {
-    if (!h || (!soa && n)) return ICE_E_ARG;
-    if (!h->pts) { h->err = "ice_allocate first"; return ICE_E_STATE; }
-    if (n > h->capacity) { h->err = "more points than the allocated capacity"; return ICE_E_CAPACITY; }
-    if (n_lines < ICE_N_LINES || n_lines > ICE_N_LINES_MAX || host_pitch < n) { h->err = "bad layout"; return ICE_E_ARG; }
-    if ((h->prm.flags & ICE_FLAG_COHESION_LINE) && n_lines < ICE_N_LINES_MAX) {
-        h->err = "ICE_FLAG_COHESION_LINE needs 20 lines";
-        return ICE_E_ARG;
-    }
-    HIPCHK(h, hipSetDevice(h->device));
-    h->nlines = (h->prm.flags & ICE_FLAG_COHESION_LINE) ? ICE_N_LINES_MAX : ICE_N_LINES;
-    if (n) HIPCHK(h, hipMemcpy2DAsync(h->pts, h->pitch * sizeof(float), soa, host_pitch * sizeof(float),
-                                      n * sizeof(float), h->nlines, hipMemcpyHostToDevice, h->stream));
-    h->n = n;
-    // the partition's disabled count starts at 0 (gpu_implementation5.cpp:137, gpu_partition.cpp:101-103)
-    const int32_t zero = 0;
-    HIPCHK(h, hipMemcpyAsync(&h->cnt->disabled, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-    float tmax = 1.f;
-    if (n) {
-        const float *th = soa + (size_t)ICE_IDX_THICKNESS * host_pitch;
-        for (uint32_t i = 0; i < n; i++) tmax = std::max(tmax, th[i]);
-    }
-    h->fix = make_fix_scale(h->prm, tmax);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->sorted = false;
-    drop_graph(h);
-    if (!h->naive()) return device_sort(h, false, nullptr);
-    return ICE_OK;
+    if (!h) return ICE_E_ARG;
+    return upload_points_d(h, 0, soa, host_pitch, n, n_lines);
 }
 
 int ice_upload_grid(IceHandle *h, const uint8_t *status, const float *normals) {
-    if (!h || !status) return ICE_E_ARG;
-    HIPCHK(h, hipSetDevice(h->device));
-    const size_t N = h->N();
-    HIPCHK(h, hipMemcpyAsync(h->status, status, N, hipMemcpyHostToDevice, h->stream));
-    if (normals) {
-        HIPCHK(h, hipMemcpyAsync(h->gline(G_NX), normals, N * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->gline(G_NY), normals + N, N * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    }
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    return ICE_OK;
+    if (!h) return ICE_E_ARG;
+    return upload_grid_d(h, 0, status, normals);
 }
 
 int ice_upload_forcing(IceHandle *h, const float *current, const float *wind) {
     if (!h) return ICE_E_ARG;
-    HIPCHK(h, hipSetDevice(h->device));
-    const size_t N = h->N();
+    return upload_forcing_d(h, 0, current, wind);
+}
+
+int ice_upload_domain(IceHandle *h, int d, const float *soa, size_t host_pitch, uint32_t n, int n_lines,
+                      const uint8_t *status, const float *current, const float *wind_stress) {
+    if (!h) return ICE_E_ARG;
     int rc;
-    if (current) {
-        HIPCHK(h, hipMemcpyAsync(h->gline(G_CX), current, N * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->gline(G_CY), current + N, N * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        bool any = false;
-        for (size_t i = 0; i < 2 * N && !any; i++) any = current[i] != 0.f;
-        h->has_current = any;
-    } else {
-        if ((rc = launch_fill(h, h->gline(G_CX), 2 * h->gpitch))) return rc;
-        h->has_current = false;
-    }
-    if (wind) {
-        HIPCHK(h, hipMemcpyAsync(h->gline(G_WX), wind, N * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->gline(G_WY), wind + N, N * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    } else {
-        if ((rc = launch_fill(h, h->gline(G_WX), 2 * h->gpitch))) return rc;
-    }
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->dp = make_dev_params(h->prm, h->has_current);
-    drop_graph(h);
-    return ICE_OK;
+    if (status && (rc = upload_grid_d(h, d, status, nullptr))) return rc;
+    if ((rc = upload_forcing_d(h, d, current, wind_stress))) return rc;
+    return upload_points_d(h, d, soa, host_pitch, n, n_lines);
 }
 
 int ice_update_constants(IceHandle *h, const IceParams *p) {
     if (!h || !p) return ICE_E_ARG;
-    if (p->GridXTotal != h->prm.GridXTotal || p->GridYTotal != h->prm.GridYTotal) {
+    if (!h->batch && (p->GridXTotal != h->dom[0].gx || p->GridYTotal != h->dom[0].gy)) {
         h->err = "grid size cannot change";
         return ICE_E_ARG;
     }
     const uint32_t old_flags = h->prm.flags;
     h->prm = *p;
     derive_params(h->prm);
-    if ((old_flags ^ h->prm.flags) & ICE_FLAG_NAIVE) h->sorted = false;
-    h->dp = make_dev_params(h->prm, h->has_current);
+    if ((old_flags ^ h->prm.flags) & ICE_FLAG_NAIVE)
+        for (auto &D : h->dom) D.sorted = false;
+    h->dp = make_dev_params(h->prm);
+    h->fix = make_fix_scale(h->prm, h->max_thickness);
     // GPU_Partition::update_constants zeroes the error word (gpu_partition.cpp:143)
-    const uint32_t zero = 0;
-    HIPCHK(h, hipMemcpyAsync(&h->cnt->err, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    for (int d = 0; d < h->ndom(); d++) {
+        const uint32_t zero = 0;
+        HIPCHK(h, hipMemcpyAsync(&h->dcnt(d)->err, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     drop_graph(h);
     return ICE_OK;
@@ -684,17 +942,17 @@ int ice_set_tile(IceHandle *h, int tw, int th) {
     if (!find_tile(tw, th)) { h->err = "unsupported tile shape"; return ICE_E_ARG; }
     h->tile_w = tw;
     h->tile_h = th;
+    h->tables_dirty = true;
     drop_graph(h);
     return ICE_OK;
 }
 
 int ice_set_option(IceHandle *h, const char *name, int value) {
     if (!h || !name) return ICE_E_ARG;
-    if (std::strcmp(name, "p2g_point_lanes") == 0) {
-        h->p2g_point_lanes = value != 0;
-    } else if (std::strcmp(name, "band") == 0) {
+    if (std::strcmp(name, "band") == 0) {
         if (value < 1 || value > 4096) { h->err = "band must be 1..4096"; return ICE_E_ARG; }
         h->band = value;
+        h->tables_dirty = true;
     } else {
         h->err = std::string("unknown option ") + name;
         return ICE_E_ARG;
@@ -706,7 +964,7 @@ int ice_set_option(IceHandle *h, const char *name, int value) {
 int ice_reset_grid(IceHandle *h) {
     if (!h) return ICE_E_ARG;
     rec(h, 0);
-    // The tiled P2G stores every node, so only the naive path clears here.
+    // The tiled P2G stores every node, so only the naive path clears (in ice_p2g).
     return ICE_OK;
 }
 
@@ -746,38 +1004,39 @@ int ice_substep(IceHandle *h, double t, int recordPQ) {
 int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
     if (!h || nsteps < 0) return ICE_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
-    if (!h->naive() && !h->sorted) {
-        int rc = device_sort(h, false, nullptr);
-        if (rc) return rc;
-    }
-    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, h->stream, h->cnt, first_step);
+    if (!h->naive())
+        for (int d = 0; d < h->ndom(); d++)
+            if (!h->dom[d].sorted) {
+                int rc = device_sort(h, d, false, nullptr);
+                if (rc) return rc;
+            }
+    int rc = h->naive() ? ICE_OK : upload_tables(h);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, h->stream, h->step_cnt(), first_step);
     LAUNCHCHK(h, "k_step_set");
     int done = 0;
-    const uint32_t resort_at = std::max<uint32_t>(256, h->n / 256);
     while (done < nsteps) {
         const int chunk = std::min(kChunk, nsteps - done);
         if (use_graph && chunk == kChunk && !h->timing) {
             if (!h->graph_exec || h->graph_chunk != chunk) {
-                int rc = build_graph(h, chunk);
-                if (rc) return rc;
+                if ((rc = build_graph(h, chunk))) return rc;
             }
             HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
         } else {
             for (int k = 0; k < chunk; k++) {
-                StepArg S{0.0, 0, &h->cnt->step, k};
-                int rc = substep_args(h, S);
-                if (rc) return rc;
+                StepArg S{0.0, 0, &h->step_cnt()->step, k};
+                if ((rc = substep_args(h, S))) return rc;
             }
-            hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, h->stream, h->cnt, chunk);
+            hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, h->stream, h->step_cnt(), chunk);
             LAUNCHCHK(h, "k_step_advance");
         }
         done += chunk;
         if (!h->naive()) {
-            Counters c;
-            int rc = read_counters(h, c);
-            if (rc) return rc;
-            if (c.n_esc > resort_at) {
-                if ((rc = device_sort(h, false, nullptr))) return rc;
+            // re-sort domains whose off-tile list grew past max(256, n/256)
+            if ((rc = read_counters(h))) return rc;
+            for (int d = 0; d < h->ndom(); d++) {
+                const uint32_t limit = std::max<uint32_t>(256, h->dom[d].n / 256);
+                if (h->host_cnt[d].n_esc > limit && (rc = device_sort(h, d, false, nullptr))) return rc;
             }
         }
     }
@@ -787,58 +1046,46 @@ int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
 
 int ice_clear_force_accumulator(IceHandle *h) {
     if (!h) return ICE_E_ARG;
-    return launch_fill(h, h->gline(G_FX), 2 * h->gpitch);
+    int rc = launch_fill(h, h->gline(G_FX), h->gpitch);
+    if (rc) return rc;
+    return launch_fill(h, h->gline(G_FY), h->gpitch);
 }
 
 int ice_download(IceHandle *h, float *soa, size_t host_pitch, int n_lines, float *forces, uint32_t *error_code,
                  int32_t *disabled_count) {
     if (!h) return ICE_E_ARG;
-    HIPCHK(h, hipSetDevice(h->device));
-    const size_t N = h->N();
-    if (soa) {
-        if (n_lines < ICE_N_LINES || n_lines > ICE_N_LINES_MAX || host_pitch < h->n) { h->err = "bad layout"; return ICE_E_ARG; }
-        const int nl = std::min(n_lines, h->nlines);
-        if (h->n)
-            HIPCHK(h, hipMemcpy2DAsync(soa, host_pitch * sizeof(float), h->pts, h->pitch * sizeof(float),
-                                       h->n * sizeof(float), nl, hipMemcpyDeviceToHost, h->stream));
-    }
-    if (forces) {
-        HIPCHK(h, hipMemcpyAsync(forces, h->gline(G_FX), N * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipMemcpyAsync(forces + N, h->gline(G_FY), N * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-    }
-    Counters c;
-    int rc = read_counters(h, c);
-    if (rc) return rc;
-    if (soa) {
-        uint32_t *u = reinterpret_cast<uint32_t *>(soa + (size_t)ICE_IDX_UTILITY * host_pitch);
-        for (uint32_t i = 0; i < h->n; i++) u[i] &= ~kInternalBits;
-    }
-    if (error_code) *error_code = c.err;
-    if (disabled_count) *disabled_count = c.disabled;
-    return ICE_OK;
+    return download_d(h, 0, soa, host_pitch, n_lines, forces, error_code, disabled_count);
+}
+
+int ice_download_domain(IceHandle *h, int d, float *soa, size_t host_pitch, int n_lines, float *forces,
+                        uint32_t *error_code, int32_t *disabled_count) {
+    if (!h) return ICE_E_ARG;
+    return download_d(h, d, soa, host_pitch, n_lines, forces, error_code, disabled_count);
 }
 
 int ice_download_grid(IceHandle *h, float *mpp, float *velocity) {
     if (!h) return ICE_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
-    const size_t N = h->N();
+    const Domain &D = h->dom[0];
     if (mpp)
-        HIPCHK(h, hipMemcpy2DAsync(mpp, N * sizeof(float), h->grid, h->gpitch * sizeof(float), N * sizeof(float), 3,
-                                   hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpy2DAsync(mpp, D.N * sizeof(float), h->gline(G_M) + D.node_base, h->gpitch * sizeof(float),
+                                   D.N * sizeof(float), 3, hipMemcpyDeviceToHost, h->stream));
     if (velocity) {
         if (h->naive()) {
             // the naive update kernel leaves velocities in (px, py)
-            HIPCHK(h, hipMemcpy2DAsync(velocity, N * sizeof(float), h->gline(G_PX), h->gpitch * sizeof(float),
-                                       N * sizeof(float), 2, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipMemcpy2DAsync(velocity, D.N * sizeof(float), h->gline(G_PX) + D.node_base,
+                                       h->gpitch * sizeof(float), D.N * sizeof(float), 2, hipMemcpyDeviceToHost,
+                                       h->stream));
         } else {
+            int rc = upload_tables(h);
+            if (rc) return rc;
             float *buf = nullptr;
-            HIPCHK(h, hipMalloc(&buf, 2 * h->gpitch * sizeof(float)));
-            hipLaunchKernelGGL(k_node_velocity_out, dim3(1024), dim3(256), 0, h->stream, h->dp, h->last_t, grid_in(h),
-                               buf, buf + h->gpitch);
+            HIPCHK(h, hipMalloc(&buf, 2 * D.N * sizeof(float)));
+            hipLaunchKernelGGL(k_node_velocity_out, dim3(1024), dim3(256), 0, h->stream, h->dp, h->last_t, h->d_desc,
+                               0, grid_lines(h), buf, buf + D.N);
             hipError_t e = hipGetLastError();
             if (e == hipSuccess)
-                e = hipMemcpy2DAsync(velocity, N * sizeof(float), buf, h->gpitch * sizeof(float), N * sizeof(float), 2,
-                                     hipMemcpyDeviceToHost, h->stream);
+                e = hipMemcpyAsync(velocity, buf, 2 * D.N * sizeof(float), hipMemcpyDeviceToHost, h->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
             (void)hipFree(buf);
             if (e != hipSuccess) { h->err = std::string("node velocity: ") + hipGetErrorString(e); return ICE_E_HIP; }
@@ -859,16 +1106,15 @@ int ice_sort(IceHandle *h, uint32_t *n_out, uint32_t *perm) {
     if (!h) return ICE_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
     uint32_t *perm_dev = nullptr;
-    const uint32_t n_before = h->n;
+    const uint32_t n_before = h->dom[0].n;
     if (perm && n_before) HIPCHK(h, hipMalloc(&perm_dev, sizeof(uint32_t) * n_before));
-    int rc = device_sort(h, true, perm_dev);
-    if (rc == ICE_OK && perm && h->n) {
-        hipError_t e = hipMemcpy(perm, perm_dev, sizeof(uint32_t) * h->n, hipMemcpyDeviceToHost);
+    int rc = device_sort(h, 0, true, perm_dev);
+    if (rc == ICE_OK && perm && h->dom[0].n) {
+        hipError_t e = hipMemcpy(perm, perm_dev, sizeof(uint32_t) * h->dom[0].n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) { h->err = hipGetErrorString(e); rc = ICE_E_HIP; }
     }
     if (perm_dev) (void)hipFree(perm_dev);
-    drop_graph(h);
-    if (n_out) *n_out = h->n;
+    if (n_out) *n_out = h->dom[0].n;
     return rc;
 }
 
@@ -908,6 +1154,83 @@ int ice_get_timings(IceHandle *h, double out[8]) {
     return ICE_OK;
 }
 
+int ice_autotune(IceHandle *h, int nsteps, int *tile_w, int *tile_h, double *best_ms) {
+    if (!h || nsteps <= 0) return ICE_E_ARG;
+    if (h->naive()) { h->err = "the naive path has no tiles"; return ICE_E_STATE; }
+    if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc;
+    for (int d = 0; d < h->ndom(); d++)
+        if (!h->dom[d].sorted && (rc = device_sort(h, d, false, nullptr))) return rc;
+    // Save everything a trial changes (points, cell starts, force lines,
+    // counters, off-tile lists) and restore it after each candidate, so tuning
+    // leaves the simulation state as it found it.
+    const size_t pts_bytes = sizeof(float) * h->pitch * h->nlines;
+    const size_t cs_bytes = sizeof(uint32_t) * h->cs_total;
+    const size_t f_bytes = sizeof(float) * h->gpitch * 2;
+    const size_t c_bytes = sizeof(Counters) * (h->dom.size() + 1);
+    const size_t o_bytes = sizeof(uint32_t) * h->total_capacity;
+    char *save = nullptr;
+    HIPCHK(h, hipMalloc(&save, pts_bytes + cs_bytes + f_bytes + c_bytes + o_bytes));
+    const std::vector<Domain> dom_save = h->dom;
+    float *const pts_save = h->pts;
+    auto copy_all = [&](bool store) -> hipError_t {
+        if (!store && h->pts != pts_save) std::swap(h->pts, h->pts_alt);   // undo a trial sort's swap
+        char *p = save;
+        void *dev[] = {h->pts, h->cs, h->gline(G_FX), h->cnt, h->offtile};
+        const size_t len[] = {pts_bytes, cs_bytes, f_bytes, c_bytes, o_bytes};
+        for (int k = 0; k < 5; k++) {
+            hipError_t e = store ? hipMemcpyAsync(p, dev[k], len[k], hipMemcpyDeviceToDevice, h->stream)
+                                 : hipMemcpyAsync(dev[k], p, len[k], hipMemcpyDeviceToDevice, h->stream);
+            if (e != hipSuccess) return e;
+            p += len[k];
+        }
+        return hipStreamSynchronize(h->stream);
+    };
+    hipError_t e = copy_all(true);
+    const int old_w = h->tile_w, old_h = h->tile_h;
+    int bw = old_w, bh = old_h;
+    double bms = 1e30;
+    const bool was_timing = h->timing;
+    h->timing = false;
+    rc = ICE_OK;
+    const int cand[][2] = {{8, 8}, {8, 16}, {16, 8}, {16, 16}, {16, 32}, {32, 16}, {32, 32}, {32, 64}, {64, 32}};
+    for (auto &c : cand) {
+        if (e != hipSuccess || rc != ICE_OK) break;
+        h->tile_w = c[0];
+        h->tile_h = c[1];
+        h->tables_dirty = true;
+        if ((rc = upload_tables(h))) break;
+        const StepArg S{0.0, 0, &h->step_cnt()->step, 0};
+        if ((rc = substep_args(h, S))) break;   // warm-up
+        if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) break;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < nsteps && rc == ICE_OK; k++) rc = substep_args(h, S);
+        if (rc) break;
+        if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) break;
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / nsteps;
+        if (ms < bms) { bms = ms; bw = c[0]; bh = c[1]; }
+        h->dom = dom_save;
+        e = copy_all(false);
+    }
+    h->timing = was_timing;
+    h->dom = dom_save;
+    if (e == hipSuccess) e = copy_all(false);
+    (void)hipFree(save);
+    const bool ok = rc == ICE_OK && e == hipSuccess;
+    h->tile_w = ok ? bw : old_w;
+    h->tile_h = ok ? bh : old_h;
+    h->tables_dirty = true;
+    drop_graph(h);
+    if (e != hipSuccess) { h->err = std::string("autotune: ") + hipGetErrorString(e); return ICE_E_HIP; }
+    if (rc) return rc;
+    if (tile_w) *tile_w = bw;
+    if (tile_h) *tile_h = bh;
+    if (best_ms) *best_ms = bms;
+    return upload_tables(h);
+}
+
 int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
     if (!ms_per_copy || reps <= 0 || bytes < 1024) return ICE_E_ARG;
     *ms_per_copy = 0.0;
@@ -943,25 +1266,18 @@ int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
 
 int ice_get_info(IceHandle *h, IceInfo *info) {
     if (!h || !info) return ICE_E_ARG;
-    info->n_points = h->n;
-    info->capacity = h->capacity;
-    info->point_pitch = h->pitch;
-    info->grid_pitch = h->gpitch;
-    info->grid_x = h->prm.GridXTotal;
-    info->grid_y = h->prm.GridYTotal;
-    info->device = h->device;
-    info->sorted = h->sorted ? 1 : 0;
-    info->steps_since_sort = h->steps_since_sort;
-    info->n_sorts = h->n_sorts;
-    info->flags = h->prm.flags;
-    info->tile_w = h->tile_w;
-    info->tile_h = h->tile_h;
-    Counters c{};
-    if (h->cnt) {
-        int rc = read_counters(h, c);
-        if (rc) return rc;
-    }
-    info->n_offtile = c.n_esc;
+    int rc = read_counters(h);
+    if (rc) return rc;
+    fill_info(h, 0, info);
+    return ICE_OK;
+}
+
+int ice_get_domain_info(IceHandle *h, int d, IceInfo *info) {
+    if (!h || !info) return ICE_E_ARG;
+    if (check_domain(h, d)) return ICE_E_ARG;
+    int rc = read_counters(h);
+    if (rc) return rc;
+    fill_info(h, d, info);
     return ICE_OK;
 }
 

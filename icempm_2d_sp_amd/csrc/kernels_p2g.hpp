@@ -1,20 +1,22 @@
-// kernels_p2g_cells.hpp — pull P2G with one lane per cell.
+// kernels_p2g.hpp — the sorted path's P2G: one workgroup per tile of TW x TH
+// nodes, pulling the points sorted into cells T +- 2, one lane per cell.
 //
-// Same tile contract as k_p2g_tiled (tile T of TW x TH nodes, points sorted
-// into cells T +- 2 are pulled, every node of T is written once by its tile),
-// but lanes map to cells, not points: a lane walks the points sorted into its
-// cell, sums their 3x3 stencil contributions in registers, and issues 27 LDS
-// atomics per cell (deterministic mode) or, by default, adds them in nine
-// barrier-separated phases of plain LDS read-add-write (one phase per stencil
-// offset: within a phase all cells of a round hit distinct nodes).  No LDS
-// atomics remain on the common path and the float sum order is fixed.
+// A lane walks the points sorted into its cell and sums their 3x3 stencil
+// contributions (simulation/kernels.cuh:39-104) in registers; unmoved points
+// need no cell-word load.  By default the workgroup then adds the cell
+// stencils in nine barrier-separated phases of plain LDS read-add-write, one
+// per stencil offset (within a phase all cells of a round hit distinct nodes),
+// so no LDS atomics remain on the common path and the float sum order is
+// fixed.  Points that changed cell since the sort add their own stencil with
+// LDS atomics in a phase of their own.  Every node of the tile is then stored
+// once: no global atomics and no grid clear.
 //
-// Points that changed cell since the sort (kMoved) may contribute around a
-// cell next to their lane's; they add their own stencil directly.  In the
-// deterministic mode the off-tile points are also added here (int64 sums do
-// not depend on order), instead of by k_p2g_offtile's float atomics.
+// Deterministic mode: the LDS sums are int64 fixed point (order-independent)
+// and the tile kernel also adds the domain's off-tile points, so results are
+// bit-identical for any schedule.  Otherwise k_p2g_offtile adds the off-tile
+// points with float global atomics after the tiles.
 #pragma once
-#include "kernels_tiled.hpp"
+#include "tile_common.hpp"
 
 namespace ice {
 
@@ -120,11 +122,17 @@ __device__ __forceinline__ void scatter_moved(const Lines &L, const DevParams &P
 }
 
 template <int TW, int TH, bool DET>
-__global__ __launch_bounds__(kTileThreads) void k_p2g_cells(DevParams P, Lines L, const uint32_t *__restrict__ cs,
-                                                            float *__restrict__ gm, float *__restrict__ gpx,
-                                                            float *__restrict__ gpy, Counters *cnt, int ntx, int nty,
-                                                            FixScale fs, const uint32_t *__restrict__ offtile,
-                                                            int band) {
+__global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const DomainDesc *__restrict__ D,
+                                                            const TileRef *__restrict__ T, int ntiles, Lines L0,
+                                                            const uint32_t *__restrict__ cs0, GridLines G0,
+                                                            Counters *cnt0, uint32_t *off0, FixScale fs) {
+    const TileCtx tc = tile_ctx<TW, TH>(P0, D, T, ntiles, L0, cs0, G0, cnt0, off0);
+    const DevParams &P = tc.P;
+    const Lines &L = tc.L;
+    const uint32_t *__restrict__ cs = tc.cs;
+    Counters *cnt = tc.cnt;
+    const uint32_t *__restrict__ offtile = tc.offtile;
+    const int x0 = tc.x0, y0 = tc.y0;
     constexpr int RW = TW + 4, RH = TH + 4;   // cells pulled
     constexpr int NCELL = RW * RH;
     constexpr int LP = TH + 1;                // padded LDS column pitch
@@ -132,9 +140,6 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_cells(DevParams P, Lines L
     using Acc = typename std::conditional<DET, unsigned long long, float>::type;
     __shared__ Acc s_acc[3 * NN];
 
-    int tx, ty;
-    tile_coords(blockIdx.x, ntx, nty, band, tx, ty);
-    const int x0 = tx * TW, y0 = ty * TH;
     for (int k = threadIdx.x; k < 3 * NN; k += kTileThreads) s_acc[k] = 0;
     __syncthreads();
 
@@ -213,15 +218,57 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_cells(DevParams P, Lines L
         if (x < P.gx && y < P.gy) {
             const size_t node = (size_t)x * P.gy + y;
             if constexpr (DET) {
-                gm[node] = (float)((double)(long long)s_acc[o] * fs.inv_m);
-                gpx[node] = (float)((double)(long long)s_acc[NN + o] * fs.inv_p);
-                gpy[node] = (float)((double)(long long)s_acc[2 * NN + o] * fs.inv_p);
+                tc.G.m[node] = (float)((double)(long long)s_acc[o] * fs.inv_m);
+                tc.G.px[node] = (float)((double)(long long)s_acc[NN + o] * fs.inv_p);
+                tc.G.py[node] = (float)((double)(long long)s_acc[2 * NN + o] * fs.inv_p);
             } else {
-                gm[node] = s_acc[o];
-                gpx[node] = s_acc[NN + o];
-                gpy[node] = s_acc[2 * NN + o];
+                tc.G.m[node] = s_acc[o];
+                tc.G.px[node] = s_acc[NN + o];
+                tc.G.py[node] = s_acc[2 * NN + o];
             }
         }
+    }
+    flush_err(cnt, err);
+}
+
+// P2G of the off-tile points: global atomics on top of the tile stores.  One
+// row of workgroups per domain (blockIdx.y).
+__global__ __launch_bounds__(256) void k_p2g_offtile(DevParams P0, const DomainDesc *__restrict__ D, Lines L0,
+                                                     GridLines G0, Counters *cnt0, const uint32_t *__restrict__ off0) {
+    const DomainDesc dd = D[blockIdx.y];
+    Counters *cnt = cnt0 + blockIdx.y;
+    const uint32_t n = cnt->n_esc;
+    DevParams P = P0;
+    P.gx = dd.gx;
+    P.gy = dd.gy;
+    const Lines L{L0.base + dd.pt_base, L0.pitch};
+    const uint32_t *offtile = off0 + dd.off_base;
+    float *gm = G0.m + dd.node_base, *gpx = G0.px + dd.node_base, *gpy = G0.py + dd.node_base;
+    uint32_t err = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t i = offtile[j];
+        const uint32_t u = L.u(ICE_IDX_UTILITY, i);
+        if (u & kDisabled) continue;
+        P2GPoint q;
+        load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
+        if (!stencil_inside(q.cx, q.cy, P)) {
+            err |= ICE_ERR_POINT_LEFT_AREA;
+            continue;
+        }
+        float wx[3], wy[3];
+        weights(q.x, wx);
+        weights(q.y, wy);
+        for (int a = -1; a <= 1; a++)
+            for (int b = -1; b <= 1; b++) {
+                float dm, dpx, dpy;
+                p2g_contrib(q, a, b, wx[a + 1] * wy[b + 1], dm, dpx, dpy);
+                const size_t node = (size_t)(q.cy + b) + (size_t)(q.cx + a) * P.gy;
+                unsafeAtomicAdd(&gm[node], dm);
+                unsafeAtomicAdd(&gpx[node], dpx);
+                unsafeAtomicAdd(&gpy[node], dpy);
+                if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+                if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+            }
     }
     flush_err(cnt, err);
 }

@@ -380,3 +380,98 @@ class Simulation:
 
     def reset_timings(self):
         self._chk(self.lib.ice_reset_timings(self.h), "ice_reset_timings")
+
+    def autotune(self, nsteps: int = 3):
+        tw, th, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        self._chk(self.lib.ice_autotune(self.h, nsteps, C.byref(tw), C.byref(th), C.byref(ms)), "ice_autotune")
+        return (tw.value, th.value), ms.value
+
+
+class BatchSimulation:
+    """Several independent scenes in one batched handle (ice_create_batch):
+    one launch per phase over all domains' tiles, HIP-graph replays."""
+
+    def __init__(self, scene_list, device: int = 0, *, flags: int | None = None, tile=None, capacity_factor=1.0):
+        self.lib = abi.load()
+        self.scenes = list(scene_list)
+        p0 = self.scenes[0].params
+        prm = p0 if flags is None else p0.replace(flags=flags)
+        for s in self.scenes:
+            if s.params.ParticleVolume != p0.ParticleVolume or s.nlines != self.scenes[0].nlines:
+                raise ValueError("batched domains share material constants, ParticleVolume and the line count")
+        self.params = prm
+        self.cp = prm.to_c()
+        nd = len(self.scenes)
+        gx = np.array([s.params.GridXTotal for s in self.scenes], np.int32)
+        gy = np.array([s.params.GridYTotal for s in self.scenes], np.int32)
+        cap = np.array([max(1, int(s.n * capacity_factor)) for s in self.scenes], np.uint32)
+        self.h = C.c_void_p()
+        abi.check(self.lib.ice_create_batch(C.byref(self.cp), device, nd, abi.ptr(gx), abi.ptr(gy), abi.ptr(cap),
+                                            C.byref(self.h)), None, "ice_create_batch")
+        if tile is not None:
+            self._chk(self.lib.ice_set_tile(self.h, tile[0], tile[1]), "ice_set_tile")
+        self.nlines = self.scenes[0].nlines
+        for d, s in enumerate(self.scenes):
+            soa = np.ascontiguousarray(s.soa, np.float32)
+            st = np.ascontiguousarray(s.status, np.uint8)
+            cur = None if s.current is None else np.ascontiguousarray(s.current, np.float32)
+            wnd = None if s.wind is None else np.ascontiguousarray(s.wind, np.float32)
+            self._chk(self.lib.ice_upload_domain(self.h, d, abi.ptr(soa), soa.shape[1], soa.shape[1], soa.shape[0],
+                                                 abi.ptr(st), abi.ptr(cur), abi.ptr(wnd)), f"ice_upload_domain {d}")
+
+    def _chk(self, rc, what):
+        abi.check(rc, self.h, what)
+
+    def close(self):
+        if self.h:
+            self.lib.ice_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, first_step: int, nsteps: int, graph: bool = True):
+        self._chk(self.lib.ice_run(self.h, first_step, nsteps, int(graph)), "ice_run")
+
+    def substep(self, t: float, record_pq: bool = False):
+        self._chk(self.lib.ice_substep(self.h, t, int(record_pq)), "ice_substep")
+
+    def sync(self):
+        self._chk(self.lib.ice_synchronize(self.h), "ice_synchronize")
+
+    def info(self, d: int = 0) -> abi.IceInfo:
+        i = abi.IceInfo()
+        self._chk(self.lib.ice_get_domain_info(self.h, d, C.byref(i)), "ice_get_domain_info")
+        return i
+
+    def n_points(self) -> int:
+        return sum(self.info(d).n_points for d in range(len(self.scenes)))
+
+    def download(self, d: int):
+        n = self.info(d).n_points
+        out = np.zeros((self.nlines, max(n, 1)), np.float32)
+        N = self.scenes[d].params.GridXTotal * self.scenes[d].params.GridYTotal
+        forces = np.zeros((2, N), np.float32)
+        err, dis = C.c_uint32(0), C.c_int32(0)
+        self._chk(self.lib.ice_download_domain(self.h, d, abi.ptr(out), out.shape[1], self.nlines, abi.ptr(forces),
+                                               C.byref(err), C.byref(dis)), "ice_download_domain")
+        return out[:, :n], forces, err.value, dis.value
+
+    def autotune(self, nsteps: int = 3):
+        tw, th, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        self._chk(self.lib.ice_autotune(self.h, nsteps, C.byref(tw), C.byref(th), C.byref(ms)), "ice_autotune")
+        return (tw.value, th.value), ms.value
+
+    def enable_timings(self, on: bool = True):
+        self._chk(self.lib.ice_enable_timings(self.h, int(on)), "ice_enable_timings")
+
+    def reset_timings(self):
+        self._chk(self.lib.ice_reset_timings(self.h), "ice_reset_timings")
+
+    def timings(self):
+        out = (C.c_double * 8)()
+        self._chk(self.lib.ice_get_timings(self.h, out), "ice_get_timings")
+        return list(out)

@@ -214,3 +214,38 @@ def dynamic_scene(seed: int = 7, gx: int = 48, gy: int = 40, *, speed: float = 3
     wind = (0.2 + 0.3 * rng.standard_normal((2, N))).astype(np.float32)
     status = border_status(gx, gy)
     return Scene(prm, soa, status, current, wind, "dynamic")
+
+
+def batch_domain(seed: int, side_cells: int, coverage: float, wind_pa: float, wind_dir: float,
+                 base_params=None) -> Scene:
+    """One domain of the batched workload: side_cells x side_cells cells,
+    ice in a seeded Voronoi floe field covering `coverage`, uniform wind stress
+    of `wind_pa` Pa toward `wind_dir` radians, 4 points per cell.  All domains
+    share ParticleVolume = h^2/4 so they can share one material block."""
+    rng = np.random.default_rng(seed)
+    gx = gy = side_cells + 1
+    ncx, ncy = gx - 1, gy - 1
+    n_floes = max(4, (ncx * ncy) // (60 * 60))
+    mask = voronoi_ice_mask(ncx, ncy, n_floes, coverage, rng)
+    x, y, _ = stratified_positions(gx, gy, rng, cell_mask=mask)
+    soa = make_points(x, y, gx, gy)
+    prm = sea_ice_params(gx, gy) if base_params is None else base_params.replace(GridXTotal=gx, GridYTotal=gy)
+    prm.ParticleVolume = prm.cellsize ** 2 / 4.0
+    N = gx * gy
+    wind = np.empty((2, N), np.float32)
+    wind[0] = wind_pa * np.cos(wind_dir)
+    wind[1] = wind_pa * np.sin(wind_dir)
+    return Scene(prm, soa, border_status(gx, gy), None, wind, f"batch{seed}")
+
+
+def config3(seed: int = 3003, n_domains: int = 64, min_side: int = 128, max_side: int = 1024) -> list:
+    """Batched plain MPM: n_domains independent domains, sides log-uniform in
+    [min_side, max_side] cells, ice coverage 25-95 % (so 16k-4M points at 4 per
+    cell), per-domain seeded wind 0.05-0.3 Pa in a random direction."""
+    rng = np.random.default_rng(seed)
+    sides = np.exp(rng.uniform(np.log(min_side), np.log(max_side), n_domains)).astype(int)
+    out = []
+    for d in range(n_domains):
+        out.append(batch_domain(seed * 1000 + d, int(sides[d]), float(rng.uniform(0.25, 0.95)),
+                                float(rng.uniform(0.05, 0.3)), float(rng.uniform(0, 2 * np.pi))))
+    return out

@@ -181,12 +181,29 @@ int ice_get_timings(IceHandle *h, double out[8]);
  * 64x32, 64x64.  No reference counterpart (the reference's only launch knobs
  * are tpb_P2G/Upd/G2P, parameters_sim.cpp:111-113). */
 int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
-/* Tuning switches of the sorted path (no reference counterpart):
- * "p2g_point_lanes" = 1 maps one lane per point in the tiled P2G instead of
- * one lane per cell (ignored in the deterministic mode); "band" (1..4096,
- * default 4) is the number of tile columns per band of the workgroup-to-tile
- * order. */
+/* Tuning switches of the sorted path (no reference counterpart): "band"
+ * (1..4096, default 4) is the number of tile columns per band of the
+ * workgroup-to-tile order. */
 int ice_set_option(IceHandle *h, const char *name, int value);
+/* Tile autotuner (no reference counterpart): times `nsteps` substeps for each
+ * tile shape from 8x8 to 64x32, restoring the simulation state after each
+ * trial, and keeps the fastest.  Outputs the chosen shape and its ms/substep. */
+int ice_autotune(IceHandle *h, int nsteps, int *tile_w, int *tile_h, double *best_ms);
+
+/* ---- batched domains (no reference counterpart; the reference runs one
+ * partition, gpu_implementation5.cpp:96).  `n_domains` independent domains of
+ * grid_x[d] x grid_y[d] nodes with room for capacity[d] points share one
+ * handle: every substep is one launch per phase over all tiles of all domains
+ * (largest domains first), and ice_run replays it from a HIP graph.  The
+ * material constants of `p` are shared; its GridX/YTotal are ignored.  The
+ * single-domain calls above act on domain 0. */
+int ice_create_batch(const IceParams *p, int device, int n_domains, const int32_t *grid_x, const int32_t *grid_y,
+                     const uint32_t *capacity, IceHandle **out);
+/* Points, status bytes (NULL keeps the current ones) and forcing of domain d. */
+int ice_upload_domain(IceHandle *h, int d, const float *soa, size_t host_pitch, uint32_t n, int n_lines,
+                      const uint8_t *status, const float *current, const float *wind_stress);
+int ice_download_domain(IceHandle *h, int d, float *soa, size_t host_pitch, int n_lines, float *boundary_forces,
+                        uint32_t *error_code, int32_t *disabled_count);
 
 /* ---- introspection */
 typedef struct IceInfo {
@@ -202,8 +219,11 @@ typedef struct IceInfo {
     uint32_t flags;
     int32_t  tile_w, tile_h;     /* tile of the sorted path */
     uint32_t n_offtile;          /* points handled off-tile until the next sort */
+    int32_t  n_domains;          /* domains in the handle */
+    int32_t  n_tiles;            /* tiles per launch, all domains */
 } IceInfo;
-int ice_get_info(IceHandle *h, IceInfo *info);
+int ice_get_info(IceHandle *h, IceInfo *info);                      /* domain 0 */
+int ice_get_domain_info(IceHandle *h, int d, IceInfo *info);
 
 /* Measurement utility (no reference counterpart): average time of a plain
  * device-to-device copy of `bytes` (read + write), over `reps` launches on

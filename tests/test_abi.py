@@ -53,7 +53,7 @@ def test_struct_layout_matches_c():
 #include "icempm.h"
 int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(IceParams), offsetof(IceParams, flags),
-         offsetof(IceParams, windStressToAccel), sizeof(IceInfo), offsetof(IceInfo, n_offtile));
+         offsetof(IceParams, windStressToAccel), sizeof(IceInfo), offsetof(IceInfo, n_tiles));
   return 0; }
 '''
     tmp = "/tmp/icempm_layout_probe"
@@ -63,7 +63,7 @@ int main(void) {
     out = subprocess.run([tmp], capture_output=True, text=True, check=True).stdout.split()
     got = [int(v) for v in out]
     exp = [C.sizeof(abi.IceParams), abi.IceParams.flags.offset, abi.IceParams.windStressToAccel.offset,
-           C.sizeof(abi.IceInfo), abi.IceInfo.n_offtile.offset]
+           C.sizeof(abi.IceInfo), abi.IceInfo.n_tiles.offset]
     assert got == exp
 
 

@@ -1,0 +1,73 @@
+"""Batched domains (ice_create_batch): one launch per phase over every domain.
+
+Pinned by: in the deterministic mode each domain of a batch is bit-identical
+to the same domain run alone; in the default mode each domain matches the
+FP64 oracle within the one-substep tolerances of tests/parity.py."""
+import numpy as np
+import pytest
+
+import oracle
+from icempm_2d_sp_amd import abi, scenes
+from icempm_2d_sp_amd.gpu import BatchSimulation, Simulation
+from parity import by_point, compare_points
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-4
+
+
+def _domains():
+    out = []
+    for k, (gx, gy) in enumerate([(70, 53), (24, 24), (41, 90), (16, 12), (33, 33)]):
+        sc = scenes.dynamic_scene(seed=40 + k, gx=gx, gy=gy, speed=400.0)
+        sc.params = sc.params.replace(ParticleVolume=0.25)
+        out.append(sc)
+    return out
+
+
+def test_batch_equals_single_domains_bitwise_det():
+    print("test_batch_equals_single_domains_bitwise_det", flush=True)
+    doms = _domains()
+    flags = doms[0].params.flags | abi.FLAG_DETERMINISTIC
+    b = BatchSimulation(doms, flags=flags)
+    b.run(0, 20, graph=True)
+    for d, sc in enumerate(doms):
+        s = Simulation(sc, flags=flags)
+        s.run(0, 20, graph=True)
+        ps, fs, es, ds = s.download()
+        pb, fb, eb, db = b.download(d)
+        np.testing.assert_array_equal(by_point(pb).view(np.uint32), by_point(ps).view(np.uint32))
+        np.testing.assert_array_equal(fb.view(np.uint32), fs.view(np.uint32))
+        assert (eb, db) == (es, ds)
+        s.close()
+    b.close()
+
+
+def test_batch_one_substep_matches_oracle():
+    print("test_batch_one_substep_matches_oracle", flush=True)
+    doms = _domains()
+    b = BatchSimulation(doms)
+    b.substep(0.0, True)
+    for d, sc in enumerate(doms):
+        pts, forces, err, dis = b.download(d)
+        o = oracle.Oracle.from_scene(sc, 64)
+        o.substep(0.0, True)
+        compare_points(pts, o.points())
+        assert err == o.error and dis == o.disabled
+    b.close()
+
+
+def test_autotune_keeps_state():
+    print("test_autotune_keeps_state", flush=True)
+    doms = _domains()
+    a = BatchSimulation(doms, flags=doms[0].params.flags | abi.FLAG_DETERMINISTIC)
+    b = BatchSimulation(doms, flags=doms[0].params.flags | abi.FLAG_DETERMINISTIC)
+    tile, ms = b.autotune(2)
+    assert tile in abi.TILES and ms > 0
+    a.run(0, 10, graph=False)
+    b.run(0, 10, graph=False)
+    for d in range(len(doms)):
+        np.testing.assert_array_equal(by_point(a.download(d)[0]).view(np.uint32),
+                                      by_point(b.download(d)[0]).view(np.uint32))
+    a.close()
+    b.close()

@@ -166,21 +166,6 @@ def test_config1_200_substeps_graph():
     sim.close()
 
 
-@pytest.mark.parametrize("lanes", ["cell", "point"])
-def test_p2g_lane_mappings_agree(lanes):
-    _print(f"test_p2g_lane_mappings_agree {lanes}")
-    sc = scenes.dynamic_scene(seed=17, speed=500.0)
-    a = Simulation(sc, flags=sc.params.flags | abi.FLAG_NAIVE)
-    b = Simulation(sc)
-    b._chk(b.lib.ice_set_option(b.h, b"p2g_point_lanes", int(lanes == "point")), "ice_set_option")
-    for k in range(12):
-        a.substep(DT * k, k == 11)
-        b.substep(DT * k, k == 11)
-    compare_points(b.download()[0], a.download()[0], max_flag_mismatch=2e-3, max_bad_frac=2e-3)
-    a.close()
-    b.close()
-
-
 def test_graph_replay_equals_eager():
     _print("test_graph_replay_equals_eager")
     sc = scenes.dynamic_scene(seed=8)
