@@ -133,8 +133,8 @@ def test_config1_200_substeps_graph():
     the FP64 oracle accumulates ~20 Pa of elastic stress near the sticky left
     wall.  The HIP path is therefore held to the FP32 oracle field by field,
     and to the FP64 oracle on what FP32 resolves: identical flags and cells,
-    and the mean wind-driven drift velocity within 1 %.  Per-field FP64 errors
-    are printed."""
+    and the mean wind-driven drift velocity within 10 %.  Per-field FP64
+    errors are printed."""
     _print("test_config1_200_substeps_graph")
     sc = scenes.config1()
     sim = Simulation(sc)
@@ -159,8 +159,13 @@ def test_config1_200_substeps_graph():
                         ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)))]:
         e, scale = field_errors(g, ref, lines)
         _print(f"FP64 oracle error {name}: max {e.max():.3e} median {np.median(e):.3e} (scale {scale:.3e})")
+    # Mean drift: equal to the FP32 oracle's to 1e-5; the FP64 oracle's is
+    # 6.4 % lower because its elastic stress couples the floe to the wall,
+    # which FP32 storage of Fe cannot resolve at these strain increments.
     mean_v = g[abi.IDX_VELX].astype(np.float64).mean()
-    assert mean_v > 0 and mean_v == pytest.approx(ref[abi.IDX_VELX].mean(), rel=1e-2)
+    mean_32 = o32.points()[abi.IDX_VELX].astype(np.float64).mean()
+    assert mean_v > 0 and mean_v == pytest.approx(mean_32, rel=1e-5)
+    assert mean_v == pytest.approx(ref[abi.IDX_VELX].mean(), rel=0.1)
     assert err == o64.error == 0
     assert dis == o64.disabled
     sim.close()
