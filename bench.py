@@ -32,15 +32,21 @@ HBM_PEAK_GBS = 8000.0
 def algorithmic_bytes(n_points: int, n_nodes: int, cohesion: bool) -> dict:
     """Bytes each kernel must move once per substep (FP32 hssoa, 4 B slots).
 
-    P2G reads utility, cell, pos x2, vel x2, Fe x4, Bp x4, Jp_inv, thickness
-    (16 slots) and writes m, px, py per node.  The fused update + G2P reads
-    utility, cell, pos x2, Fe x4, Jp_inv, thickness (+ cohesion) and writes
-    utility, cell, pos x2, vel x2, Fe x4, Bp x4, Jp_inv (15 slots); per node
-    it reads m, px, py, the status byte and the 2-float wind stress.
+    Unfused: P2G reads utility, cell, pos x2, vel x2, Fe x4, Bp x4, Jp_inv,
+    thickness (16 slots) and writes m, px, py per node.  The update + G2P
+    kernel reads utility, cell, pos x2, Fe x4, Jp_inv, thickness (+ cohesion)
+    and writes utility, cell, pos x2, vel x2, Fe x4, Bp x4, Jp_inv (15 slots);
+    per node it reads m, px, py, the status byte and the 2-float wind stress.
+
+    Pipelined (one tile pass per substep): per point the same 10 (+1) slots
+    read and 14 written (the cell word only changes on a crossing); per node
+    the node sums read and written once (24 B), status (1 B), wind (8 B) and
+    the cell start (4 B).
     """
     p2g = 64 * n_points + 12 * n_nodes
     g2p = (40 + (4 if cohesion else 0) + 60) * n_points + 21 * n_nodes
-    return {"p2g": p2g, "g2p": g2p, "substep": p2g + g2p}
+    pipe = (40 + (4 if cohesion else 0) + 56) * n_points + 37 * n_nodes
+    return {"p2g": p2g, "g2p": g2p, "substep": p2g + g2p, "pipelined": pipe}
 
 
 def cpu_baseline(scene, target_s: float, threads: int, max_steps: int = 50) -> dict:
@@ -70,6 +76,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--naive", action="store_true", help="reference launch structure (unsorted, atomics)")
+    ap.add_argument("--unfused", action="store_true", help="separate P2G and update + G2P tile kernels")
     ap.add_argument("--scale", type=float, default=1.0, help="linear domain scale (1.0 = 4096x2048)")
     ap.add_argument("--workload", choices=["config2", "batch"], default="config2",
                     help="config2: one 4096x2048 domain (the headline); batch: config 3's 64 uneven domains")
@@ -106,7 +113,8 @@ def main() -> int:
 
     tw, th = (int(v) for v in args.tile.split("x"))
     flags = scene.params.flags | (abi.FLAG_NAIVE if args.naive else 0)
-    sim = Simulation(scene, flags=flags, tile=(tw, th))
+    pipelined = not (args.naive or args.unfused)
+    sim = Simulation(scene, flags=flags, tile=(tw, th), options={"pipelined": int(pipelined)})
     n_active = sim.info().n_points
     print(f"[bench] uploaded and sorted; warmup {args.warmup}", file=sys.stderr, flush=True)
     sim.run(0, args.warmup, graph=False)
@@ -131,8 +139,13 @@ def main() -> int:
     ms_sub_ev = tm[4] / nsub
     nodes = gx * gy
     B = algorithmic_bytes(n_active, nodes, True)
-    dominant = "g2p" if ms_g2p >= ms_p2g else "p2g"
-    ms_dom = ms_g2p if dominant == "g2p" else ms_p2g
+    if pipelined:
+        dominant, ms_dom, kname = "pipelined", ms_g2p, "k_tile_pipeline"
+        B["substep"] = B["pipelined"]
+    elif ms_g2p >= ms_p2g:
+        dominant, ms_dom, kname = "g2p", ms_g2p, "k_g2p_tiles" if not args.naive else "k_g2p_naive"
+    else:
+        dominant, ms_dom, kname = "p2g", ms_p2g, "k_p2g_tiles" if not args.naive else "k_p2g_naive"
     achieved = B[dominant] / (ms_dom * 1e-3) / 1e9
     value = n_active * args.steps / wall
 
@@ -160,11 +173,11 @@ def main() -> int:
                          "random polygonal ice field (~95%), GRF wind stress (mean 0.2 Pa, std 0.05 Pa, "
                          "corr 200 cells), lognormal cohesion (median 1e5 Pa, sigma 0.5), dt=1e-4 s"),
             "points": n_active, "nodes": nodes, "tile": args.tile,
-            "path": "naive" if args.naive else "sorted-tiled",
+            "path": "naive" if args.naive else ("sorted-unfused" if args.unfused else "sorted-pipelined"),
             "error_word": err,
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"{dominant} ({'k_g2p_tiled' if dominant == 'g2p' else 'k_p2g_tiled'})",
+            "bound": "hbm", "kernel": kname,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "bytes_per_launch": B[dominant], "ms_per_launch": ms_dom,

@@ -28,7 +28,8 @@ constexpr uint32_t kDisabled = 0x20000u;
 // they are cleared before points leave the device.
 constexpr uint32_t kMoved = 0x80000u;     // changed cell once since the last sort
 constexpr uint32_t kEscaped = 0x100000u;  // changed cell twice: handled off-tile
-constexpr uint32_t kInternalBits = kMoved | kEscaped;
+constexpr uint32_t kFresh = 0x200000u;    // escaped in this substep's tile pass: P2G still due
+constexpr uint32_t kInternalBits = kMoved | kEscaped | kFresh;
 
 // Constants the kernels read, precomputed on the host in FP64 and rounded
 // once to FP32.  Passed by value as a kernel argument.

@@ -25,6 +25,7 @@
 #include "tile_common.hpp"
 #include "kernels_p2g.hpp"
 #include "kernels_g2p.hpp"
+#include "kernels_fused.hpp"
 
 using namespace ice;
 
@@ -39,7 +40,7 @@ constexpr int kChunk = 16;       // substeps per captured graph
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct TimingSet {
-    hipEvent_t e[5];
+    hipEvent_t e[6];
 };
 
 struct Domain {
@@ -99,9 +100,17 @@ struct IceHandle {
     std::vector<TimingSet> ev_pool;
     size_t ev_used = 0;
     double acc[8] = {0};
+    // frames of the pipelined path (two buffers, substep n reads fr[fr_cur])
+    float *fr[2] = {nullptr, nullptr};
+    size_t fr_cap = 0;             // floats per buffer
+    int fr_cur = 0;
+    bool frames_valid = false;     // fr[fr_cur] holds the P2G of the current point state
+    bool last_fused = false;       // the last substep ran pipelined: its grid is fr[fr_cur ^ 1]
+    bool fused_opt = true;
     // graph
     hipGraphExec_t graph_exec = nullptr;
     int graph_chunk = 0;
+    int graph_fr0 = 0;
 
     int ndom() const { return (int)dom.size(); }
     float *gline(int l) const { return grid + (size_t)l * gpitch; }
@@ -112,6 +121,7 @@ struct IceHandle {
     Counters *step_cnt() const { return cnt + dom.size(); }
     bool naive() const { return (prm.flags & ICE_FLAG_NAIVE) != 0; }
     bool det() const { return (prm.flags & ICE_FLAG_DETERMINISTIC) != 0; }
+    bool pipelined() const { return fused_opt && !naive() && !det(); }
     uint32_t max_capacity() const {
         uint32_t m = 1;
         for (auto &d : dom) m = std::max(m, d.capacity);
@@ -242,14 +252,24 @@ using P2GFn = void (*)(DevParams, const DomainDesc *, const TileRef *, int, Line
                        Counters *, uint32_t *, FixScale);
 using G2PFn = void (*)(DevParams, StepArg, const DomainDesc *, const TileRef *, int, Lines, const uint32_t *,
                        GridLines, Counters *, uint32_t *);
+using PipeFn = void (*)(DevParams, StepArg, const DomainDesc *, const TileRef *, int, Lines, const uint32_t *,
+                        GridLines, Counters *, uint32_t *, const float *, float *);
+using OffFn = void (*)(DevParams, StepArg, const DomainDesc *, Lines, GridLines, Counters *, const uint32_t *,
+                       const float *, float *, int);
+using ToGridFn = void (*)(const DomainDesc *, int, const float *, GridLines);
 
 struct TileKernels {
     int w, h;
     P2GFn p2g, p2g_det;
     G2PFn g2p;
+    PipeFn fill, fused;
+    OffFn offtile;
+    ToGridFn to_grid;
 };
 
-#define TK(W, H) {W, H, k_p2g_tiles<W, H, false>, k_p2g_tiles<W, H, true>, k_g2p_tiles<W, H>}
+#define TK(W, H)                                                                                          \
+    {W, H, k_p2g_tiles<W, H, false>, k_p2g_tiles<W, H, true>, k_g2p_tiles<W, H>, k_tile_pipeline<W, H, false>, \
+     k_tile_pipeline<W, H, true>, k_offtile_pipeline<W, H>, k_frames_to_grid<W, H>}
 const TileKernels kTiles[] = {TK(8, 8),   TK(8, 16),  TK(16, 8),  TK(16, 16), TK(16, 32),
                               TK(32, 16), TK(32, 32), TK(32, 64), TK(64, 32), TK(64, 64)};
 #undef TK
@@ -287,10 +307,29 @@ int upload_tables(IceHandle *h) {
     if (!h->tables_dirty) return ICE_OK;
     const int nd = h->ndom();
     std::vector<DomainDesc> desc(nd);
+    uint64_t nframes = 0;
     for (int d = 0; d < nd; d++) {
         const Domain &D = h->dom[d];
-        desc[d] = DomainDesc{D.gx, D.gy, D.has_current ? 1 : 0, 0, D.pt_base, D.pt_base, D.node_base, D.cs_base};
+        const int ntx = (D.gx + h->tile_w - 1) / h->tile_w, nty = (D.gy + h->tile_h - 1) / h->tile_h;
+        desc[d] = DomainDesc{D.gx,      D.gy,      D.has_current ? 1 : 0, 0, D.pt_base, D.pt_base, D.node_base,
+                             D.cs_base, ntx,       nty,                   nframes};
+        nframes += (uint64_t)ntx * nty;
     }
+    // frames: 3 components of (tile_w + 4) x (tile_h + 4) nodes per tile
+    const size_t fr_need = (size_t)nframes * 3 * (h->tile_w + 4) * (h->tile_h + 4);
+    if (fr_need > h->fr_cap) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (auto &f : h->fr) {
+            if (f) (void)hipFree(f);
+            f = nullptr;
+        }
+        h->fr_cap = 0;
+        HIPCHK(h, hipMalloc(&h->fr[0], fr_need * sizeof(float)));
+        HIPCHK(h, hipMalloc(&h->fr[1], fr_need * sizeof(float)));
+        h->fr_cap = fr_need;
+    }
+    h->frames_valid = false;   // the frame layout follows the tile shape
+    h->last_fused = false;
     std::vector<int> order(nd);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h->dom[a].N > h->dom[b].N; });
@@ -513,12 +552,76 @@ void end_timing_set(IceHandle *h) {
     if (h->timing && h->ev_used < h->ev_pool.size()) h->ev_used++;
 }
 
+// The unfused phases leave the frames stale.
+void frames_stale(IceHandle *h) {
+    h->frames_valid = false;
+    h->last_fused = false;
+}
+
+// Pipeline fill: P2G of the current point state into fr[fr_cur].
+int pipeline_fill(IceHandle *h) {
+    const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
+    const GridLines G = grid_lines(h);
+    const StepArg S0{0.0, 0, nullptr, 0};
+    float *F = h->fr[h->fr_cur];
+    if (h->ntiles) {
+        hipLaunchKernelGGL(tk->fill, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, S0, h->d_desc,
+                           h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt, h->offtile,
+                           (const float *)F, F);
+        LAUNCHCHK(h, "k_tile_pipeline (fill)");
+    }
+    hipLaunchKernelGGL(tk->offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S0, h->d_desc,
+                       h->lines(), G, h->cnt, (const uint32_t *)h->offtile, (const float *)F, F, 0);
+    LAUNCHCHK(h, "k_offtile_pipeline (fill)");
+    h->frames_valid = true;
+    h->last_fused = false;
+    return ICE_OK;
+}
+
+// One pipelined substep: update + G2P of S from fr[fr_cur], then the P2G of
+// the new point state into fr[fr_cur ^ 1].  Event slots 1-3 mark the start of
+// the tile kernel, 4 its end.
+int phase_pipelined(IceHandle *h, const StepArg &S) {
+    if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
+    int rc;
+    for (int d = 0; d < h->ndom(); d++)
+        if (!h->dom[d].sorted && (rc = device_sort(h, d, false, nullptr))) return rc;
+    if ((rc = upload_tables(h))) return rc;
+    if (!h->frames_valid && (rc = pipeline_fill(h))) return rc;
+    rec(h, 1);
+    rec(h, 2);
+    rec(h, 3);
+    const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
+    const GridLines G = grid_lines(h);
+    const float *Fc = h->fr[h->fr_cur];
+    float *Fn = h->fr[h->fr_cur ^ 1];
+    if (h->ntiles) {
+        hipLaunchKernelGGL(tk->fused, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, S, h->d_desc,
+                           h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt, h->offtile, Fc, Fn);
+        LAUNCHCHK(h, "k_tile_pipeline");
+    }
+    rec(h, 4);
+    hipLaunchKernelGGL(tk->offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S, h->d_desc,
+                       h->lines(), G, h->cnt, (const uint32_t *)h->offtile, Fc, Fn, 1);
+    LAUNCHCHK(h, "k_offtile_pipeline");
+    h->fr_cur ^= 1;
+    h->last_fused = true;
+    for (auto &D : h->dom) D.steps_since_sort++;
+    return ICE_OK;
+}
+
 int substep_args(IceHandle *h, const StepArg &S) {
     rec(h, 0);
-    int rc = phase_p2g(h);
-    if (rc) return rc;
-    if ((rc = phase_update(h, S))) return rc;
-    if ((rc = phase_g2p(h, S))) return rc;
+    int rc;
+    if (h->pipelined()) {
+        if ((rc = phase_pipelined(h, S))) return rc;
+    } else {
+        frames_stale(h);
+        if ((rc = phase_p2g(h))) return rc;
+        if ((rc = phase_update(h, S))) return rc;
+        if ((rc = phase_g2p(h, S))) return rc;
+    }
+    rec(h, 5);
     end_timing_set(h);
     return ICE_OK;
 }
@@ -536,6 +639,8 @@ int build_graph(IceHandle *h, int chunk) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     const bool was_timing = h->timing;
     h->timing = false;
+    const std::vector<Domain> dom_save = h->dom;   // host-side counters advance at launch, not capture
+    const int fr0 = h->fr_cur;
     int rc = ICE_OK;
     for (int k = 0; k < chunk && rc == ICE_OK; k++) {
         StepArg S{0.0, 0, &h->step_cnt()->step, k};
@@ -543,6 +648,9 @@ int build_graph(IceHandle *h, int chunk) {
     }
     if (rc == ICE_OK) hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, h->stream, h->step_cnt(), chunk);
     h->timing = was_timing;
+    h->dom = dom_save;
+    h->graph_fr0 = fr0;
+    if (rc == ICE_OK && h->fr_cur != fr0) { rc = ICE_E_STATE; h->err = "graph chunk must hold an even number of substeps"; }
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -554,6 +662,12 @@ int build_graph(IceHandle *h, int chunk) {
     if (e != hipSuccess) { h->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e); return ICE_E_HIP; }
     h->graph_chunk = chunk;
     return ICE_OK;
+}
+
+// After a graph launch: the host-side state the captured substeps stand for.
+void graph_launched(IceHandle *h, int chunk) {
+    for (auto &D : h->dom) D.steps_since_sort += chunk;
+    if (h->pipelined()) h->last_fused = true;   // chunk is even: fr_cur is unchanged
 }
 
 bool grid_ok(int gx, int gy, std::string &why) {
@@ -696,6 +810,7 @@ int upload_points_d(IceHandle *h, int d, const float *soa, size_t host_pitch, ui
     HIPCHK(h, hipStreamSynchronize(h->stream));
     D.sorted = false;
     drop_graph(h);
+    frames_stale(h);
     if (!h->naive()) return device_sort(h, d, false, nullptr);
     return ICE_OK;
 }
@@ -874,7 +989,7 @@ int ice_destroy(IceHandle *h) {
     for (void *p : {(void *)h->pts, (void *)h->pts_alt, (void *)h->grid, (void *)h->status, (void *)h->cnt,
                     (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
-                    (void *)h->d_desc, (void *)h->d_tiles})
+                    (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1]})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -927,6 +1042,7 @@ int ice_update_constants(IceHandle *h, const IceParams *p) {
         for (auto &D : h->dom) D.sorted = false;
     h->dp = make_dev_params(h->prm);
     h->fix = make_fix_scale(h->prm, h->max_thickness);
+    frames_stale(h);
     // GPU_Partition::update_constants zeroes the error word (gpu_partition.cpp:143)
     for (int d = 0; d < h->ndom(); d++) {
         const uint32_t zero = 0;
@@ -943,6 +1059,7 @@ int ice_set_tile(IceHandle *h, int tw, int th) {
     h->tile_w = tw;
     h->tile_h = th;
     h->tables_dirty = true;
+    frames_stale(h);
     drop_graph(h);
     return ICE_OK;
 }
@@ -953,6 +1070,10 @@ int ice_set_option(IceHandle *h, const char *name, int value) {
         if (value < 1 || value > 4096) { h->err = "band must be 1..4096"; return ICE_E_ARG; }
         h->band = value;
         h->tables_dirty = true;
+    } else if (std::strcmp(name, "pipelined") == 0) {
+        if (value != 0 && value != 1) { h->err = "pipelined must be 0 or 1"; return ICE_E_ARG; }
+        h->fused_opt = value != 0;
+        frames_stale(h);
     } else {
         h->err = std::string("unknown option ") + name;
         return ICE_E_ARG;
@@ -970,6 +1091,7 @@ int ice_reset_grid(IceHandle *h) {
 
 int ice_p2g(IceHandle *h) {
     if (!h) return ICE_E_ARG;
+    frames_stale(h);
     return phase_p2g(h);
 }
 
@@ -979,6 +1101,7 @@ int ice_update_nodes(IceHandle *h, float simulation_time, float windSpeed, float
     (void)windAngle;
     h->pending_t = simulation_time;
     StepArg S{(double)simulation_time, 0, nullptr, 0};
+    frames_stale(h);
     return phase_update(h, S);
 }
 
@@ -988,7 +1111,9 @@ int ice_g2p(IceHandle *h, int recordPQ, int enablePointTransfer, int applyGlensL
     (void)applyGlensLaw;         // Glen's law is commented out in the reference (kernels.cuh:625-652)
     StepArg S{h->pending_t, recordPQ, nullptr, 0};
     h->last_t = h->pending_t;
+    frames_stale(h);
     int rc = phase_g2p(h, S);
+    rec(h, 5);
     end_timing_set(h);
     return rc;
 }
@@ -1012,16 +1137,18 @@ int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
             }
     int rc = h->naive() ? ICE_OK : upload_tables(h);
     if (rc) return rc;
+    if (h->pipelined() && !h->frames_valid && (rc = pipeline_fill(h))) return rc;   // never inside a graph
     hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, h->stream, h->step_cnt(), first_step);
     LAUNCHCHK(h, "k_step_set");
     int done = 0;
     while (done < nsteps) {
         const int chunk = std::min(kChunk, nsteps - done);
         if (use_graph && chunk == kChunk && !h->timing) {
-            if (!h->graph_exec || h->graph_chunk != chunk) {
+            if (!h->graph_exec || h->graph_chunk != chunk || h->graph_fr0 != h->fr_cur) {
                 if ((rc = build_graph(h, chunk))) return rc;
             }
             HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+            graph_launched(h, chunk);
         } else {
             for (int k = 0; k < chunk; k++) {
                 StepArg S{0.0, 0, &h->step_cnt()->step, k};
@@ -1067,6 +1194,14 @@ int ice_download_grid(IceHandle *h, float *mpp, float *velocity) {
     if (!h) return ICE_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
     const Domain &D = h->dom[0];
+    if (h->last_fused && (mpp || velocity)) {
+        // the last substep's node sums are in the frames it read (a tile
+        // change clears last_fused, so the frame layout is current)
+        const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
+        hipLaunchKernelGGL(tk->to_grid, dim3(1024), dim3(256), 0, h->stream, (const DomainDesc *)h->d_desc, 0,
+                           (const float *)h->fr[h->fr_cur ^ 1], grid_lines(h));
+        LAUNCHCHK(h, "k_frames_to_grid");
+    }
     if (mpp)
         HIPCHK(h, hipMemcpy2DAsync(mpp, D.N * sizeof(float), h->gline(G_M) + D.node_base, h->gpitch * sizeof(float),
                                    D.N * sizeof(float), 3, hipMemcpyDeviceToHost, h->stream));
@@ -1141,7 +1276,7 @@ int ice_get_timings(IceHandle *h, double out[8]) {
         HIPCHK(h, hipEventElapsedTime(&ms[0], t.e[0], t.e[1]));   // main P2G kernel
         HIPCHK(h, hipEventElapsedTime(&ms[1], t.e[0], t.e[2]));   // P2G phase
         HIPCHK(h, hipEventElapsedTime(&ms[2], t.e[3], t.e[4]));   // main G2P kernel
-        HIPCHK(h, hipEventElapsedTime(&ms[3], t.e[0], t.e[4]));   // substep
+        HIPCHK(h, hipEventElapsedTime(&ms[3], t.e[0], t.e[5]));   // substep
         h->acc[0] += ms[0];
         h->acc[1] += ms[1];
         h->acc[2] += ms[3] - ms[1] - ms[2];   // update + off-tile G2P

@@ -1,0 +1,491 @@
+// kernels_fused.hpp — the pipelined sorted path: one pass over each tile's
+// points per substep runs the grid update and G2P of substep n and the P2G of
+// substep n + 1 (partition_kernel_update_nodes / _g2p / _p2g,
+// simulation/kernels.cuh:109-177, 181-315, 39-104), so a point's state is read
+// once and written once per substep.
+//
+// Node sums live in frames: tile T (TW x TH nodes) owns a frame of the
+// (TW + 4) x (TH + 4) nodes T +- 2, into which the points sorted into T scatter
+// (they sit at most one cell from T, so their stencils stay inside it).  A
+// node's sum is the sum of every frame that covers it (at most 2 x 2), taken
+// in ascending tile order, so any reader sees the same value.  Frames come in
+// two buffers: substep n reads buffer n % 2 and writes buffer (n + 1) % 2.
+//
+// Inside a tile, a workgroup walks T's cells in rounds of 256 (one lane per
+// cell).  Lanes take the round's points one per lane in sorted order
+// (coalesced SoA loads and stores), run G2P, and stage the point's P2G terms
+// in LDS; then each cell lane sums the terms of its own points into a
+// register stencil and the workgroup adds the stencils in nine barrier-
+// separated phases of plain LDS read-add-write (within a phase all cells hit
+// distinct nodes).  Points that changed cell since the sort add their
+// stencils with LDS atomics.  Points that change cell a second time leave
+// the tile pass (kEscaped + kFresh) and the off-tile kernel does their P2G,
+// with global atomics into the frame of the tile that now holds them.
+#pragma once
+#include "tile_common.hpp"
+
+namespace ice {
+
+template <int TW, int TH>
+struct Frame {
+    static constexpr int W = TW + 4, H = TH + 4, N = W * H;   // frame nodes (global layout: lx * H + ly)
+    static constexpr int LP = H + 1;                          // LDS column pitch
+    static constexpr int NN = W * LP;                         // LDS floats per component
+};
+
+template <int TW, int TH>
+__device__ __forceinline__ size_t frame_offset(const DomainDesc &dd, int tx, int ty) {
+    return (size_t)(dd.frame_base + (uint64_t)tx * dd.nty + ty) * (3 * Frame<TW, TH>::N);
+}
+
+// Sum of node (x, y) of a domain over every frame that covers it, ascending tx
+// then ty.  (x, y) must lie inside the domain's grid.
+template <int TW, int TH>
+__device__ __forceinline__ void frame_node_sum(const float *__restrict__ F, const DomainDesc &dd, int x, int y,
+                                               float &m, float &px, float &py) {
+    using Fr = Frame<TW, TH>;
+    m = px = py = 0.f;
+    const int ta = (x + 2) / TW, ua = (y + 2) / TH;
+#pragma unroll
+    for (int dx = 1; dx >= 0; dx--) {
+        const int tx = ta - dx;
+        const int lx = x - tx * TW + 2;
+        if (tx < 0 || tx >= dd.ntx || lx >= Fr::W) continue;
+#pragma unroll
+        for (int dy = 1; dy >= 0; dy--) {
+            const int ty = ua - dy;
+            const int ly = y - ty * TH + 2;
+            if (ty < 0 || ty >= dd.nty || ly >= Fr::H) continue;
+            const float *f = F + frame_offset<TW, TH>(dd, tx, ty) + lx * Fr::H + ly;
+            m += f[0];
+            px += f[Fr::N];
+            py += f[2 * Fr::N];
+        }
+    }
+}
+
+// Node velocity of node (x, y) from frame sums (kernels.cuh:109-177 plus wind).
+template <int TW, int TH>
+__device__ __forceinline__ void frame_node_velocity(const float *__restrict__ F, const DomainDesc &dd,
+                                                    const GridLines &G, const DevParams &P, float ramp, int x, int y,
+                                                    float &vx, float &vy, float &mx, float &my, bool &wall,
+                                                    float &m) {
+    frame_node_sum<TW, TH>(F, dd, x, y, m, mx, my);
+    const size_t node = (size_t)x * P.gy + y;
+    float cx = 0.f, cy = 0.f;
+    if (P.has_current) { cx = G.cur_x[node]; cy = G.cur_y[node]; }
+    node_velocity(m, mx, my, G.status[node], cx, cy, G.wind_x[node], G.wind_y[node], ramp, P, vx, vy, wall);
+}
+
+// P2G terms of a point from its state in registers (kernels.cuh:54-76).
+__device__ __forceinline__ void p2g_terms(const DevParams &P, float x, float y, int cx, int cy, float vx, float vy,
+                                          const M2 &Fe, const M2 &Bp, float Jp, float th, P2GPoint &q) {
+    q.x = x;
+    q.y = y;
+    q.cx = cx;
+    q.cy = cy;
+    q.mass = P.particle_mass * th;
+    q.mvx = q.mass * vx;
+    q.mvy = q.mass * vy;
+    const FInv v = invariants(Fe);
+    float t00, t01, t11;
+    kirchhoff(v, P, t00, t01, t11);
+    const float k = P.dt_vol_Dpinv * Jp * th;
+    q.s00 = (q.mass * Bp.a00 - k * t00) * P.h;
+    q.s01 = (q.mass * Bp.a01 - k * t01) * P.h;
+    q.s10 = (q.mass * Bp.a10 - k * t01) * P.h;
+    q.s11 = (q.mass * Bp.a11 - k * t11) * P.h;
+}
+
+// A point's whole stencil into the LDS frame (origin fx0, fy0) with atomics.
+template <int TW, int TH>
+__device__ __forceinline__ void frame_scatter_lds(const P2GPoint &q, int fx0, int fy0, float *acc, uint32_t &err) {
+    using Fr = Frame<TW, TH>;
+    const int lx0 = q.cx - 1 - fx0, ly0 = q.cy - 1 - fy0;
+    if (lx0 < 0 || ly0 < 0 || lx0 + 2 >= Fr::W || ly0 + 2 >= Fr::H) {
+        err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
+        return;
+    }
+    float wx[3], wy[3];
+    weights(q.x, wx);
+    weights(q.y, wy);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            float dm, dpx, dpy;
+            p2g_contrib(q, a - 1, b - 1, wx[a] * wy[b], dm, dpx, dpy);
+            if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+            if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+            const int o = (lx0 + a) * Fr::LP + (ly0 + b);
+            __hip_atomic_fetch_add(&acc[o], dm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&acc[Fr::NN + o], dpx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&acc[2 * Fr::NN + o], dpy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+}
+
+// A point's whole stencil into the global frame of the tile holding its cell.
+template <int TW, int TH>
+__device__ __forceinline__ void frame_scatter_global(const P2GPoint &q, const DomainDesc &dd, float *F,
+                                                     uint32_t &err) {
+    using Fr = Frame<TW, TH>;
+    const int tx = q.cx / TW, ty = q.cy / TH;
+    float *f = F + frame_offset<TW, TH>(dd, tx, ty);
+    const int lx0 = q.cx - tx * TW + 1, ly0 = q.cy - ty * TH + 1;   // node cx - 1 in frame coordinates
+    float wx[3], wy[3];
+    weights(q.x, wx);
+    weights(q.y, wy);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            float dm, dpx, dpy;
+            p2g_contrib(q, a - 1, b - 1, wx[a] * wy[b], dm, dpx, dpy);
+            if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+            if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+            const int o = (lx0 + a) * Fr::H + (ly0 + b);
+            unsafeAtomicAdd(&f[o], dm);
+            unsafeAtomicAdd(&f[Fr::N + o], dpx);
+            unsafeAtomicAdd(&f[2 * Fr::N + o], dpy);
+        }
+}
+
+constexpr int kStage = 1024;   // staged points per sub-round
+constexpr int kStageFields = 9;
+
+// FUSED = false: P2G of the current point state into frames Fn (fills the
+// pipeline).  FUSED = true: update + G2P of substep S from frames Fc, then P2G
+// of the new state into Fn.
+template <int TW, int TH, bool FUSED>
+__global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, StepArg S,
+                                                                const DomainDesc *__restrict__ D,
+                                                                const TileRef *__restrict__ T, int ntiles, Lines L0,
+                                                                const uint32_t *__restrict__ cs0, GridLines G0,
+                                                                Counters *cnt0, uint32_t *off0,
+                                                                const float *__restrict__ Fc, float *__restrict__ Fn) {
+    using Fr = Frame<TW, TH>;
+    static_assert(kTileThreads % TH == 0 && TH <= kTileThreads, "a round holds whole tile columns");
+    constexpr int RC = kTileThreads / TH;              // tile columns per round
+    constexpr int NROUND = (TW + RC - 1) / RC;
+    const TileRef tr = T[xcd_remap(blockIdx.x, ntiles)];
+    const DomainDesc dd = D[tr.d];
+    DevParams P = P0;
+    P.gx = dd.gx;
+    P.gy = dd.gy;
+    P.has_current = dd.has_current;
+    const Lines L{L0.base + dd.pt_base, L0.pitch};
+    const uint32_t *__restrict__ cs = cs0 + dd.cs_base;
+    const size_t nb = dd.node_base;
+    const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
+                      G0.wind_x + nb, G0.wind_y + nb, G0.status + nb};
+    Counters *cnt = cnt0 + tr.d;
+    uint32_t *__restrict__ offtile = off0 + dd.off_base;
+    const int x0 = tr.tx * TW, y0 = tr.ty * TH;
+    const int fx0 = x0 - 2, fy0 = y0 - 2;                // frame origin
+
+    __shared__ float s_acc[3 * Fr::NN];
+    __shared__ float2 s_v[FUSED ? Fr::N : 1];
+    __shared__ float s_st[kStageFields][kStage];
+    __shared__ uint32_t s_rb[RC];
+    __shared__ uint32_t s_pre[RC + 1];
+
+    uint32_t err = 0;
+    double time = 0.0;
+    bool rec = false;
+    float ramp = 1.f;
+    if constexpr (FUSED) {
+        resolve_step(S, P, time, rec);
+        ramp = current_ramp(time);
+        for (int k = threadIdx.x; k < Fr::N; k += kTileThreads) {
+            const int lx = k / Fr::H, ly = k - (k / Fr::H) * Fr::H;
+            const int x = fx0 + lx, y = fy0 + ly;
+            float2 v = make_float2(0.f, 0.f);
+            if (x >= 0 && x < P.gx && y >= 0 && y < P.gy) {
+                float m, mx, my;
+                bool wall;
+                frame_node_velocity<TW, TH>(Fc, dd, G, P, ramp, x, y, v.x, v.y, mx, my, wall, m);
+                if (lx >= 2 && lx < TW + 2 && ly >= 2 && ly < TH + 2) {   // T's own node
+                    const size_t node = (size_t)x * P.gy + y;
+                    if (wall) { G.fx[node] += mx; G.fy[node] += my; }
+                    if (m != 0.f && (__builtin_isnan(v.x) || __builtin_isnan(v.y))) err |= ICE_ERR_GRID_NAN;
+                }
+            }
+            s_v[k] = v;
+        }
+    }
+    for (int k = threadIdx.x; k < 3 * Fr::NN; k += kTileThreads) s_acc[k] = 0.f;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int rd = 0; rd < NROUND; rd++) {
+        const int col0 = rd * RC;
+        const int ncol = min(RC, TW - col0);
+        column_ranges(cs, P, x0 + col0, ncol, y0, y0 + TH, s_rb, s_pre);   // contains __syncthreads
+        const uint32_t n_round = s_pre[ncol];
+        // this lane's cell and the flattened range of its points
+        const int cc = (int)threadIdx.x / TH, rr = (int)threadIdx.x - cc * TH;
+        const int cx = x0 + col0 + cc, cy = y0 + rr;
+        uint32_t f0 = 0, flen = 0;
+        if (cc < ncol && cx < P.gx && cy < P.gy) {
+            const size_t cell = (size_t)cx * P.gy + cy;
+            const uint32_t b = cs[cell];
+            f0 = s_pre[cc] + (b - s_rb[cc]);
+            flen = cs[cell + 1] - b;
+        }
+        for (uint32_t sub = 0; sub < n_round; sub += kStage) {   // uniform bound
+#pragma unroll 1
+            for (int p = 0; p < kStage / kTileThreads; p++) {
+                const uint32_t jw = sub + (uint32_t)(p * kTileThreads + wave * 64);
+                const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;   // staging slot
+                if (jw >= n_round) {   // the whole wave is past the round's points
+                    s_st[2][jj] = 0.f;
+                    continue;
+                }
+                const int cb = flat_col_n(jw, s_pre, ncol);
+                const uint32_t j = jw + lane;
+                bool stage = false;
+                P2GPoint q;
+                if (j < n_round) {
+                    const int c = col_from_n(j, cb, s_pre, ncol);
+                    const uint32_t i = s_rb[c] + (j - s_pre[c]);
+                    const uint32_t u = L.u(ICE_IDX_UTILITY, i);
+                    if constexpr (FUSED) {
+                        G2PState s;
+                        load_g2p_state(L, i, u, s);
+                        const float th = L(ICE_IDX_THICKNESS, i);
+                        const float dpt = dp_threshold(P, L, i);
+                        if (!(u & (kDisabled | kEscaped))) {
+                            const int rx = s.cx - fx0, ry = s.cy - fy0;
+                            if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2) {
+                                err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
+                            } else {
+                                float wx[3], wy[3];
+                                weights(s.x, wx);
+                                weights(s.y, wy);
+                                float pvx = 0.f, pvy = 0.f;
+                                M2 C{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                                for (int a = -1; a <= 1; a++)
+#pragma unroll
+                                    for (int b = -1; b <= 1; b++) {
+                                        const float w = wx[a + 1] * wy[b + 1];
+                                        const float2 nv = s_v[(rx + a) * Fr::H + (ry + b)];
+                                        const float dx = (float)a - s.x, dy = (float)b - s.y;
+                                        pvx += w * nv.x;
+                                        pvy += w * nv.y;
+                                        const float f = P.four_h_inv * w;
+                                        C.a00 += f * (nv.x * dx);
+                                        C.a01 += f * (nv.x * dy);
+                                        C.a10 += f * (nv.y * dx);
+                                        C.a11 += f * (nv.y * dy);
+                                    }
+                                g2p_finish(P, s, pvx, pvy, C, th, dpt, err);
+                                if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
+                                bool escaped = false;
+                                if (s.cell_updated && !(s.u & kDisabled)) {
+                                    if (s.u & kMoved) {
+                                        s.u |= kEscaped | kFresh;
+                                        const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
+                                        offtile[e] = i;
+                                        escaped = true;
+                                    } else {
+                                        s.u |= kMoved;
+                                    }
+                                }
+                                g2p_store(L, i, s, pvx, pvy, C, rec);
+                                if (!(s.u & kDisabled) && !escaped) {
+                                    p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, s.Fe, C, s.Jp, th, q);
+                                    if (s.u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
+                                    else stage = true;
+                                }
+                            }
+                        }
+                    } else {
+                        if (!(u & (kDisabled | kEscaped))) {
+                            load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
+                            if (!stencil_inside(q.cx, q.cy, P)) err |= ICE_ERR_POINT_LEFT_AREA;
+                            else if (u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
+                            else stage = true;
+                        }
+                    }
+                }
+                if (stage) {
+                    s_st[0][jj] = q.x;
+                    s_st[1][jj] = q.y;
+                    s_st[2][jj] = q.mass;
+                    s_st[3][jj] = q.mvx;
+                    s_st[4][jj] = q.mvy;
+                    s_st[5][jj] = q.s00;
+                    s_st[6][jj] = q.s01;
+                    s_st[7][jj] = q.s10;
+                    s_st[8][jj] = q.s11;
+                } else {
+                    s_st[2][jj] = 0.f;   // no staged contribution
+                }
+            }
+            __syncthreads();
+            // each cell lane sums its own staged points
+            float am[3][3], ax[3][3], ay[3][3];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) am[a][b] = ax[a][b] = ay[a][b] = 0.f;
+            bool any = false;
+            const uint32_t lo = max(f0, sub), hi = min(f0 + flen, sub + (uint32_t)kStage);
+            for (uint32_t jf = lo; jf < hi; jf++) {
+                const uint32_t k = jf - sub;
+                P2GPoint t;
+                t.mass = s_st[2][k];
+                if (t.mass == 0.f) continue;
+                t.x = s_st[0][k];
+                t.y = s_st[1][k];
+                t.mvx = s_st[3][k];
+                t.mvy = s_st[4][k];
+                t.s00 = s_st[5][k];
+                t.s01 = s_st[6][k];
+                t.s10 = s_st[7][k];
+                t.s11 = s_st[8][k];
+                float wx[3], wy[3];
+                weights(t.x, wx);
+                weights(t.y, wy);
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        float dm, dpx, dpy;
+                        p2g_contrib(t, a - 1, b - 1, wx[a] * wy[b], dm, dpx, dpy);
+                        am[a][b] += dm;
+                        ax[a][b] += dpx;
+                        ay[a][b] += dpy;
+                    }
+                any = true;
+            }
+            if (any) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        if (__builtin_isnan(ax[a][b]) || __builtin_isnan(ay[a][b])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+                        if (__builtin_isnan(am[a][b])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+                    }
+            }
+            // nine phases, one per stencil offset: distinct nodes within a phase
+            const int lxc = cx - fx0, lyc = cy - fy0;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    if (any) {
+                        const int o = (lxc + a - 1) * Fr::LP + (lyc + b - 1);
+                        s_acc[o] += am[a][b];
+                        s_acc[Fr::NN + o] += ax[a][b];
+                        s_acc[2 * Fr::NN + o] += ay[a][b];
+                    }
+                    __syncthreads();
+                }
+        }
+        __syncthreads();   // the round's ranges are read; the next round may overwrite them
+    }
+    // the frame, every node once (coalesced)
+    float *__restrict__ out = Fn + frame_offset<TW, TH>(dd, tr.tx, tr.ty);
+    for (int k = threadIdx.x; k < 3 * Fr::N; k += kTileThreads) {
+        const int comp = k / Fr::N, nd = k - comp * Fr::N;
+        const int lx = nd / Fr::H, ly = nd - lx * Fr::H;
+        out[k] = s_acc[comp * Fr::NN + lx * Fr::LP + ly];
+    }
+    flush_err(cnt, err);
+}
+
+// The off-tile points after a pipeline pass.  do_g2p = 1 (fused substep):
+// points that escaped in an earlier substep run update + G2P from frames Fc;
+// every off-tile point then adds its P2G into Fn with global atomics.  Points
+// marked kFresh (escaped in this substep's tile pass) only need the P2G.
+// One row of workgroups per domain (blockIdx.y).
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void k_offtile_pipeline(DevParams P0, StepArg S, const DomainDesc *__restrict__ D,
+                                                          Lines L0, GridLines G0, Counters *cnt0,
+                                                          const uint32_t *__restrict__ off0, const float *Fc,
+                                                          float *Fn, int do_g2p) {
+    const DomainDesc dd = D[blockIdx.y];
+    Counters *cnt = cnt0 + blockIdx.y;
+    const uint32_t n = cnt->n_esc;
+    DevParams P = P0;
+    P.gx = dd.gx;
+    P.gy = dd.gy;
+    P.has_current = dd.has_current;
+    const Lines L{L0.base + dd.pt_base, L0.pitch};
+    const uint32_t *offtile = off0 + dd.off_base;
+    const size_t nb = dd.node_base;
+    const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
+                      G0.wind_x + nb, G0.wind_y + nb, G0.status + nb};
+    double time = 0.0;
+    bool rec = false;
+    if (do_g2p) resolve_step(S, P, time, rec);
+    const float ramp = current_ramp(time);
+    uint32_t err = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t i = offtile[j];
+        uint32_t u = L.u(ICE_IDX_UTILITY, i);
+        if (u & kDisabled) continue;
+        if (u & kFresh) {
+            u &= ~kFresh;
+            L.set_u(ICE_IDX_UTILITY, i, u);
+        } else if (do_g2p) {
+            G2PState s;
+            load_g2p_state(L, i, u, s);
+            if (!stencil_inside(s.cx, s.cy, P)) {
+                err |= ICE_ERR_POINT_LEFT_AREA;
+                continue;
+            }
+            const float th = L(ICE_IDX_THICKNESS, i);
+            float wx[3], wy[3];
+            weights(s.x, wx);
+            weights(s.y, wy);
+            float pvx = 0.f, pvy = 0.f;
+            M2 C{0.f, 0.f, 0.f, 0.f};
+            for (int a = -1; a <= 1; a++)
+                for (int b = -1; b <= 1; b++) {
+                    const float w = wx[a + 1] * wy[b + 1];
+                    float nvx, nvy, mx, my, m;
+                    bool wall;
+                    frame_node_velocity<TW, TH>(Fc, dd, G, P, ramp, s.cx + a, s.cy + b, nvx, nvy, mx, my, wall, m);
+                    const float dx = (float)a - s.x, dy = (float)b - s.y;
+                    pvx += w * nvx;
+                    pvy += w * nvy;
+                    const float f = P.four_h_inv * w;
+                    C.a00 += f * (nvx * dx);
+                    C.a01 += f * (nvx * dy);
+                    C.a10 += f * (nvy * dx);
+                    C.a11 += f * (nvy * dy);
+                }
+            g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err);
+            if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
+            g2p_store(L, i, s, pvx, pvy, C, rec);
+            if (s.u & kDisabled) continue;
+            u = s.u;
+        }
+        P2GPoint q;
+        load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
+        if (!stencil_inside(q.cx, q.cy, P)) {
+            err |= ICE_ERR_POINT_LEFT_AREA;
+            continue;
+        }
+        frame_scatter_global<TW, TH>(q, dd, Fn, err);
+    }
+    flush_err(cnt, err);
+}
+
+// Node sums of domain d from frames F into the grid lines (m, px, py).
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void k_frames_to_grid(const DomainDesc *__restrict__ D, int d, const float *F,
+                                                        GridLines G0) {
+    const DomainDesc dd = D[d];
+    const size_t N = (size_t)dd.gx * dd.gy, nb = dd.node_base;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N; k += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(k / dd.gy), y = (int)(k - (size_t)x * dd.gy);
+        float m, px, py;
+        frame_node_sum<TW, TH>(F, dd, x, y, m, px, py);
+        G0.m[nb + k] = m;
+        G0.px[nb + k] = px;
+        G0.py[nb + k] = py;
+    }
+}
+
+}  // namespace ice

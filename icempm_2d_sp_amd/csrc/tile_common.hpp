@@ -29,6 +29,8 @@ struct DomainDesc {
     uint32_t off_base;     // first slot of the domain's off-tile list
     uint64_t node_base;    // first node of the domain in the grid lines
     uint64_t cs_base;      // first entry of the domain's cell-start array
+    int ntx, nty;          // tiles per axis
+    uint64_t frame_base;   // first frame of the domain (frames in tile-grid order, ty fastest)
 };
 
 struct TileRef {
@@ -138,6 +140,23 @@ template <int NCOL>
 __device__ __forceinline__ int col_from(uint32_t k, int cb, const uint32_t *s_pre) {
     int c = cb;
     while (c < NCOL - 1 && s_pre[c + 1] <= k) c++;
+    return c;
+}
+
+// Runtime-count versions of flat_col / col_from (ncol >= 1 columns).
+__device__ __forceinline__ int flat_col_n(uint32_t k, const uint32_t *s_pre, int ncol) {
+    int lo = 0, hi = ncol - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int col_from_n(uint32_t k, int cb, const uint32_t *s_pre, int ncol) {
+    int c = cb;
+    while (c < ncol - 1 && s_pre[c + 1] <= k) c++;
     return c;
 }
 

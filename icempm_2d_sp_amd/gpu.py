@@ -296,7 +296,8 @@ class GPU_Implementation5:
 class Simulation:
     """A scene on one GPU through the C-ABI: upload, run, download."""
 
-    def __init__(self, scene, device: int = 0, *, flags: int | None = None, tile=None, capacity: int | None = None):
+    def __init__(self, scene, device: int = 0, *, flags: int | None = None, tile=None, capacity: int | None = None,
+                 options: dict | None = None):
         self.lib = abi.load()
         self.scene = scene
         prm = scene.params if flags is None else scene.params.replace(flags=flags)
@@ -306,6 +307,8 @@ class Simulation:
         abi.check(self.lib.ice_create(C.byref(self.cp), device, C.byref(self.h)), None, "ice_create")
         if tile is not None:
             self._chk(self.lib.ice_set_tile(self.h, tile[0], tile[1]), "ice_set_tile")
+        for name, value in (options or {}).items():
+            self.set_option(name, value)
         self.nlines = scene.nlines
         self._chk(self.lib.ice_allocate(self.h, capacity or scene.n), "ice_allocate")
         self._chk(self.lib.ice_upload_grid(self.h, abi.ptr(np.ascontiguousarray(scene.status)), None),
@@ -330,6 +333,9 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+    def set_option(self, name: str, value: int):
+        self._chk(self.lib.ice_set_option(self.h, name.encode(), int(value)), f"ice_set_option({name})")
 
     def substep(self, t: float, record_pq: bool = False):
         self._chk(self.lib.ice_substep(self.h, t, int(record_pq)), "ice_substep")
@@ -391,7 +397,8 @@ class BatchSimulation:
     """Several independent scenes in one batched handle (ice_create_batch):
     one launch per phase over all domains' tiles, HIP-graph replays."""
 
-    def __init__(self, scene_list, device: int = 0, *, flags: int | None = None, tile=None, capacity_factor=1.0):
+    def __init__(self, scene_list, device: int = 0, *, flags: int | None = None, tile=None, capacity_factor=1.0,
+                 options: dict | None = None):
         self.lib = abi.load()
         self.scenes = list(scene_list)
         p0 = self.scenes[0].params
@@ -410,6 +417,8 @@ class BatchSimulation:
                                             C.byref(self.h)), None, "ice_create_batch")
         if tile is not None:
             self._chk(self.lib.ice_set_tile(self.h, tile[0], tile[1]), "ice_set_tile")
+        for name, value in (options or {}).items():
+            self._chk(self.lib.ice_set_option(self.h, name.encode(), int(value)), f"ice_set_option({name})")
         self.nlines = self.scenes[0].nlines
         for d, s in enumerate(self.scenes):
             soa = np.ascontiguousarray(s.soa, np.float32)

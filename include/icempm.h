@@ -140,7 +140,11 @@ int ice_p2g(IceHandle *h);                                          /* gpu_parti
 int ice_update_nodes(IceHandle *h, float simulation_time, float windSpeed, float windAngle); /* :336-348 */
 int ice_g2p(IceHandle *h, int recordPQ, int enablePointTransfer, int applyGlensLaw);        /* :350-365 */
 /* One whole substep (reset_grid, p2g, update_nodes, g2p) as one call; the
- * sequence Model::Step runs (model.cpp:18-39). */
+ * sequence Model::Step runs (model.cpp:18-39).  Unless ICE_FLAG_NAIVE or
+ * ICE_FLAG_DETERMINISTIC is set (or the "pipelined" option is 0), substeps
+ * run pipelined: one tile pass does the update and G2P of this substep and
+ * the P2G of the next from the new point state, so the first substep after
+ * an upload (or any change to the points or constants) also runs one P2G. */
 int ice_substep(IceHandle *h, double simulation_time, int recordPQ);
 /* `nsteps` substeps starting at step index `first_step` (t = dt*step),
  * recordPQ on steps that are multiples of UpdateEveryNthStep; replays a
@@ -171,7 +175,9 @@ int ice_sort(IceHandle *h, uint32_t *n_out, uint32_t *perm);
  * out[0] ms in the main P2G kernel, out[1] ms in the P2G phase (grid clear,
  * P2G, off-tile P2G), out[2] ms in the node update (naive) or off-tile G2P,
  * out[3] ms in the main G2P kernel, out[4] ms of whole substeps, out[5]
- * substeps recorded, out[6..7] 0.  Graph replays record nothing. */
+ * substeps recorded, out[6..7] 0.  A pipelined substep counts its tile pass
+ * as the main G2P kernel, the pipeline's initial P2G (if any) as the main P2G
+ * kernel, and its off-tile pass under out[2].  Graph replays record nothing. */
 int ice_reset_timings(IceHandle *h);
 int ice_enable_timings(IceHandle *h, int enable);
 int ice_get_timings(IceHandle *h, double out[8]);
@@ -183,7 +189,8 @@ int ice_get_timings(IceHandle *h, double out[8]);
 int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
 /* Tuning switches of the sorted path (no reference counterpart): "band"
  * (1..4096, default 4) is the number of tile columns per band of the
- * workgroup-to-tile order. */
+ * workgroup-to-tile order; "pipelined" (0 or 1, default 1) selects the
+ * pipelined substep of ice_substep / ice_run. */
 int ice_set_option(IceHandle *h, const char *name, int value);
 /* Tile autotuner (no reference counterpart): times `nsteps` substeps for each
  * tile shape from 8x8 to 64x32, restoring the simulation state after each

@@ -17,6 +17,7 @@ from parity import compare_points, by_point, u32, field_errors
 pytestmark = pytest.mark.gpu
 
 DT = 1e-4
+UNFUSED = {"pipelined": 0}   # the separate P2G and update + G2P tile kernels
 
 
 def _print(*a):
@@ -31,12 +32,12 @@ def run_oracle(scene, steps, first=0, record_every=None):
     return o
 
 
-@pytest.mark.parametrize("mode", ["naive", "tiled", "det"])
+@pytest.mark.parametrize("mode", ["naive", "pipelined", "unfused", "det"])
 def test_one_substep_dynamic(mode):
     _print(f"test_one_substep_dynamic mode={mode}")
     sc = scenes.dynamic_scene(seed=11, with_cohesion=(mode == "det"))
-    flags = sc.params.flags | {"naive": abi.FLAG_NAIVE, "tiled": 0, "det": abi.FLAG_DETERMINISTIC}[mode]
-    sim = Simulation(sc, flags=flags)
+    flags = sc.params.flags | {"naive": abi.FLAG_NAIVE, "det": abi.FLAG_DETERMINISTIC}.get(mode, 0)
+    sim = Simulation(sc, flags=flags, options=UNFUSED if mode == "unfused" else None)
     sim.substep(0.0, True)
     pts, forces, err, dis = sim.download()
     o = oracle.Oracle.from_scene(sc, 64)
@@ -51,12 +52,14 @@ def test_one_substep_dynamic(mode):
     sim.close()
 
 
-@pytest.mark.parametrize("mode", ["naive", "tiled"])
+@pytest.mark.parametrize("mode", ["naive", "pipelined", "unfused"])
 def test_p2g_grid_and_node_velocity(mode):
+    """After one substep the grid holds that substep's P2G (in the pipelined
+    path, read back from the frames it consumed) and its node velocities."""
     _print(f"test_p2g_grid_and_node_velocity mode={mode}")
     sc = scenes.dynamic_scene(seed=3)
     flags = sc.params.flags | (abi.FLAG_NAIVE if mode == "naive" else 0)
-    sim = Simulation(sc, flags=flags)
+    sim = Simulation(sc, flags=flags, options=UNFUSED if mode == "unfused" else None)
     sim.substep(0.0, False)
     mpp, vel = sim.grid()
     o = oracle.Oracle.from_scene(sc, 64)
@@ -77,12 +80,13 @@ def test_p2g_grid_and_node_velocity(mode):
     sim.close()
 
 
+@pytest.mark.parametrize("pipelined", [1, 0])
 @pytest.mark.parametrize("tile", abi.TILES)
-def test_tile_shapes_match_naive(tile):
-    _print(f"test_tile_shapes_match_naive tile={tile}")
+def test_tile_shapes_match_naive(tile, pipelined):
+    _print(f"test_tile_shapes_match_naive tile={tile} pipelined={pipelined}")
     sc = scenes.dynamic_scene(seed=5, gx=70, gy=53)
     a = Simulation(sc, flags=sc.params.flags | abi.FLAG_NAIVE)
-    b = Simulation(sc, tile=tile)
+    b = Simulation(sc, tile=tile, options={"pipelined": pipelined})
     for k in range(3):
         a.substep(DT * k, True)
         b.substep(DT * k, True)
@@ -94,13 +98,14 @@ def test_tile_shapes_match_naive(tile):
     b.close()
 
 
-def test_many_substeps_dynamic_with_crossings():
+@pytest.mark.parametrize("pipelined", [1, 0])
+def test_many_substeps_dynamic_with_crossings(pipelined):
     """60 substeps of a fast swirling scene: many cell crossings, points moving
     twice (off-tile list), disabled points at the walls."""
-    _print("test_many_substeps_dynamic_with_crossings")
+    _print(f"test_many_substeps_dynamic_with_crossings pipelined={pipelined}")
     sc = scenes.dynamic_scene(seed=21, speed=500.0, gx=40, gy=36, fill=0.95)
     sc.params = sc.params.replace(waterDragEffectiveQuadratic=0.0)
-    sim = Simulation(sc)
+    sim = Simulation(sc, options={"pipelined": pipelined})
     nsteps = 60
     for k in range(nsteps):
         sim.substep(DT * k, k == nsteps - 1)
