@@ -95,11 +95,13 @@ def main() -> int:
     if args.copy_only > 0:
         import ctypes as C
         nbytes = int(args.copy_only * 1e9)
-        ms = C.c_double(0)
-        abi.check(abi.load().ice_copy_bench(0, nbytes, 20, C.byref(ms)), None, "ice_copy_bench")
-        gbs = 2 * nbytes / (ms.value * 1e-3) / 1e9
-        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms.value, "GBs_read_plus_write": gbs,
-                          "frac_of_peak": gbs / HBM_PEAK_GBS}), flush=True)
+        out = {"bytes": nbytes}
+        for mode, name, moved in [(0, "copy", 2 * nbytes), (1, "read", nbytes), (2, "write", nbytes)]:
+            ms = C.c_double(0)
+            abi.check(abi.load().ice_stream_bench(0, nbytes, 20, mode, C.byref(ms)), None, "ice_stream_bench")
+            gbs = moved / (ms.value * 1e-3) / 1e9
+            out[name] = {"ms": ms.value, "GBs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
+        print(json.dumps(out), flush=True)
         return 0
 
     if args.workload == "batch":

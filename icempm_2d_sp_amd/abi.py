@@ -97,6 +97,7 @@ SIGNATURES = {
     "ice_set_option": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "ice_get_info": (C.c_int, [_P, C.POINTER(IceInfo)]),
     "ice_copy_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "ice_stream_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "ice_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "ice_create_batch": (C.c_int, [C.POINTER(IceParams), C.c_int, C.c_int, _P, _P, _P, C.POINTER(_P)]),
     "ice_upload_domain": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.c_uint32, C.c_int, _P, _P, _P]),

@@ -39,6 +39,8 @@ struct DevParams {
     float dt_h_inv;             // dt / h
     float four_h_inv;           // 4 / h
     float mu, kappa, half_kappa;
+    float inv_2mu;              // 1 / (2 mu)
+    float inv_half_kappa;       // 2 / kappa
     float particle_mass;        // ParticleVolume * IceDensity
     float dt_vol_Dpinv;
     float vmax2_01;             // 0.1 * vmax^2
@@ -60,6 +62,17 @@ struct M2 {
     float a00, a01, a10, a11;
 };
 
+// 1/b from the hardware reciprocal and one Newton step (within 1 ulp), in
+// place of the 10-instruction IEEE division sequence.
+__device__ __forceinline__ float rcp_nr(float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.f);
+    return __builtin_fmaf(r, e, r);
+}
+
+// Hardware square root (within 1 ulp).
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ void weights(float x, float w[3]) {
     // quadratic B-spline weights of nodes -1, 0, +1 (kernels.cuh:323-332)
     float v0 = 0.5f - x, v2 = x + 0.5f;
@@ -73,6 +86,7 @@ struct FInv {
     float E, H, D, G;   // conformal / anti-conformal parts
     float Q, R;         // sigma0 = Q + R, sigma1 = Q - R
     float J, Jm1;       // det F and det F - 1
+    float iJ;           // 1 / J
 };
 
 __device__ __forceinline__ FInv invariants(const M2 &F) {
@@ -82,10 +96,11 @@ __device__ __forceinline__ FInv invariants(const M2 &F) {
     v.H = 0.5f * (F.a10 - F.a01);
     v.D = 0.5f * (e00 - e11);
     v.G = 0.5f * (F.a01 + F.a10);
-    v.Q = __builtin_sqrtf(v.E * v.E + v.H * v.H);
-    v.R = __builtin_sqrtf(v.D * v.D + v.G * v.G);
+    v.Q = sqrt_hw(v.E * v.E + v.H * v.H);
+    v.R = sqrt_hw(v.D * v.D + v.G * v.G);
     v.Jm1 = e00 + e11 + (e00 * e11 - F.a01 * F.a10);
     v.J = 1.f + v.Jm1;
+    v.iJ = rcp_nr(v.J);
     return v;
 }
 
@@ -94,7 +109,7 @@ __device__ __forceinline__ FInv invariants(const M2 &F) {
 __device__ __forceinline__ void kirchhoff(const FInv &v, const DevParams &P, float &t00, float &t01, float &t11) {
     const float d = 2.f * (v.E * v.D - v.H * v.G);       // (b00 - b11) / 2
     const float b01 = 2.f * (v.E * v.G + v.D * v.H);
-    const float muJ = P.mu / v.J;
+    const float muJ = P.mu * v.iJ;
     const float pi = P.half_kappa * v.Jm1 * (v.J + 1.f);
     t00 = muJ * d + pi;
     t11 = -muJ * d + pi;
@@ -105,7 +120,7 @@ __device__ __forceinline__ void kirchhoff(const FInv &v, const DevParams &P, flo
 // with |dev(F F^T)| = 2 sqrt2 Q R.
 __device__ __forceinline__ void compute_pq(const FInv &v, const DevParams &P, float &p, float &q) {
     p = -P.half_kappa * v.Jm1 * (v.J + 1.f);
-    q = 4.f * P.mu * v.Q * v.R / v.J;
+    q = 4.f * P.mu * v.Q * v.R * v.iJ;
 }
 
 // CheckIfPointIsInsideFailureSurface (kernels.cuh:349-369)
@@ -121,7 +136,7 @@ __device__ __forceinline__ bool outside_failure_surface(float p, float q, const 
 __device__ __forceinline__ float q_yield_at(float p, float dpt, const DevParams &P) {
     // min(elliptic cap, linear DP), 0 above pmax (kernels.cuh:419-429)
     if (!(p < P.pmax)) return 0.f;
-    const float q_fs = P.ellipse_c * __builtin_sqrtf((P.pmax - p) * (p - P.pmin));
+    const float q_fs = P.ellipse_c * sqrt_hw((P.pmax - p) * (p - P.pmin));
     const float q_dp = fmaxf(0.f, (p - dpt) * P.tan_phi);
     return fminf(q_fs, q_dp);
 }
@@ -130,9 +145,9 @@ __device__ __forceinline__ float q_yield_at(float p, float dpt, const DevParams 
 __device__ __forceinline__ M2 rebuild(const FInv &v, float s0, float s1) {
     const float sb = 0.5f * (s0 + s1), sd = 0.5f * (s0 - s1);
     float rc = 1.f, rs = 0.f;                  // Rot = I when Q = 0 (helper_math.cuh:31-39)
-    if (v.Q != 0.f) { const float iq = 1.f / v.Q; rc = v.E * iq; rs = v.H * iq; }
+    if (v.Q != 0.f) { const float iq = rcp_nr(v.Q); rc = v.E * iq; rs = v.H * iq; }
     float fc = 0.f, fs = 0.f;
-    if (v.R != 0.f) { const float ir = 1.f / v.R; fc = v.D * ir; fs = v.G * ir; }
+    if (v.R != 0.f) { const float ir = rcp_nr(v.R); fc = v.D * ir; fs = v.G * ir; }
     M2 F;
     F.a00 = sb * rc + sd * fc;
     F.a01 = -sb * rs + sd * fs;
@@ -147,24 +162,24 @@ __device__ __forceinline__ int return_map(const FInv &v, float p, float q, float
                                           const DevParams &P, M2 &Fe, float &Jp_inv) {
     if (p < dpt) {
         if (Jp_inv < 0.1f) {
-            const float s = __builtin_sqrtf(v.J);
+            const float s = sqrt_hw(v.J);
             Fe = rebuild(v, s, s);
             return 0;
         }
         Fe = rebuild(v, 1.f, 1.f);
-        Jp_inv /= v.J;
+        Jp_inv *= v.iJ;
         return 1;
     }
     if (!(q > q_yield_at(p, dpt, P))) return 2;
     const float p_ridge_max = P.ridge * thickness * Jp_inv;
     const float p1 = fminf(p, p_ridge_max);
     const float q1 = q_yield_at(p1, dpt, P);
-    const float Je_new = __builtin_sqrtf(1.f - 2.f * p1 / P.kappa);
+    const float Je_new = sqrt_hw(1.f - p1 * P.inv_half_kappa);
     // normalized(s_hat) = sign(J) (1, -1)/sqrt2 when sigma0^2 != sigma1^2, else 0
     float t = 0.f;
-    if (v.Q * v.R != 0.f) t = q1 * fabsf(v.J) / (2.f * P.mu);
-    Fe = rebuild(v, __builtin_sqrtf(Je_new + t), __builtin_sqrtf(Je_new - t));
-    Jp_inv *= Je_new / v.J;
+    if (v.Q * v.R != 0.f) t = q1 * fabsf(v.J) * P.inv_2mu;
+    Fe = rebuild(v, sqrt_hw(Je_new + t), sqrt_hw(Je_new - t));
+    Jp_inv *= Je_new * v.iJ;
     return 3;
 }
 
@@ -180,7 +195,7 @@ __device__ __forceinline__ void node_velocity(float m, float mx, float my, uint8
                                               float wx, float wy, float ramp, const DevParams &P, float &vx,
                                               float &vy, bool &wall) {
     if (m == 0.f) { vx = mx; vy = my; wall = false; return; }   // untouched node keeps (px, py)
-    const float im = 1.f / m;
+    const float im = rcp_nr(m);
     vx = mx * im;
     vy = my * im;
     wall = status != 100;
@@ -188,7 +203,7 @@ __device__ __forceinline__ void node_velocity(float m, float mx, float my, uint8
         vx = 0.f; vy = 0.f;
     } else {
         const float ux = cx * ramp - vx, uy = cy * ramp - vy;
-        const float k = fminf(P.kL + P.kQ * __builtin_sqrtf(ux * ux + uy * uy), 0.1f);
+        const float k = fminf(P.kL + P.kQ * sqrt_hw(ux * ux + uy * uy), 0.1f);
         vx += k * ux;
         vy += k * uy;
         vx += P.wind_dt * wx;

@@ -179,6 +179,8 @@ DevParams make_dev_params(const IceParams &p) {
     d.mu = (float)p.mu;
     d.kappa = (float)p.kappa;
     d.half_kappa = (float)(0.5 * p.kappa);
+    d.inv_2mu = (float)(0.5 / p.mu);
+    d.inv_half_kappa = (float)(2.0 / p.kappa);
     d.particle_mass = (float)p.ParticleMass;
     d.dt_vol_Dpinv = (float)p.dt_vol_Dpinv;
     d.vmax2_01 = (float)(0.1 * p.vmax * p.vmax);
@@ -725,7 +727,8 @@ int alloc_points(IceHandle *h) {
         D.sorted = false;
         base = round_up(base + std::max<uint32_t>(D.capacity, 1), 64);
     }
-    if (base >= 0xffffffffull) { h->err = "total point capacity exceeds 2^32"; return ICE_E_CAPACITY; }
+    // kernels address a line slot by a 32-bit byte offset
+    if (base >= (1ull << 30)) { h->err = "total point capacity exceeds 2^30 slots"; return ICE_E_CAPACITY; }
     h->total_capacity = (uint32_t)base;
     h->pitch = base;
     const size_t bytes = sizeof(float) * h->pitch * ICE_N_LINES_MAX;
@@ -1366,11 +1369,12 @@ int ice_autotune(IceHandle *h, int nsteps, int *tile_w, int *tile_h, double *bes
     return upload_tables(h);
 }
 
-int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
-    if (!ms_per_copy || reps <= 0 || bytes < 1024) return ICE_E_ARG;
-    *ms_per_copy = 0.0;
+int ice_stream_bench(int device, size_t bytes, int reps, int mode, double *ms_per_pass) {
+    if (!ms_per_pass || reps <= 0 || bytes < 1024 || mode < 0 || mode > 2) return ICE_E_ARG;
+    *ms_per_pass = 0.0;
     if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice failed"; return ICE_E_DEVICE; }
     const size_t n4 = bytes / 16;
+    const unsigned blocks = 8192, threads = 256;
     float4 *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1379,16 +1383,22 @@ int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
         if (e != hipSuccess && rc == ICE_OK) { g_create_error = hipGetErrorString(e); rc = ICE_E_HIP; }
         return rc == ICE_OK;
     };
-    if (ok(hipMalloc(&a, n4 * 16)) && ok(hipMalloc(&b, n4 * 16)) && ok(hipStreamCreate(&st)) &&
+    // b: the copy destination, or the read kernel's sink (one float per thread)
+    const size_t b_bytes = mode == 1 ? (size_t)blocks * threads * sizeof(float) : n4 * 16;
+    auto launch = [&]() {
+        if (mode == 0) hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(threads), 0, st, a, b, n4);
+        else if (mode == 1) hipLaunchKernelGGL(k_read4, dim3(blocks), dim3(threads), 0, st, a, n4, (float *)b);
+        else hipLaunchKernelGGL(k_write4, dim3(blocks), dim3(threads), 0, st, a, n4);
+    };
+    if (ok(hipMalloc(&a, n4 * 16)) && ok(hipMalloc(&b, b_bytes)) && ok(hipStreamCreate(&st)) &&
         ok(hipEventCreate(&e0)) && ok(hipEventCreate(&e1)) && ok(hipMemsetAsync(a, 0, n4 * 16, st))) {
-        const unsigned blocks = 4096;
-        hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, st, a, b, n4);   // warm-up
+        launch();   // warm-up
         ok(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, st, a, b, n4);
+        for (int r = 0; r < reps; r++) launch();
         ok(hipGetLastError());
         ok(hipEventRecord(e1, st));
         float ms = 0.f;
-        if (ok(hipEventSynchronize(e1)) && ok(hipEventElapsedTime(&ms, e0, e1))) *ms_per_copy = ms / reps;
+        if (ok(hipEventSynchronize(e1)) && ok(hipEventElapsedTime(&ms, e0, e1))) *ms_per_pass = ms / reps;
     }
     if (st) (void)hipStreamSynchronize(st);
     if (e0) (void)hipEventDestroy(e0);
@@ -1397,6 +1407,10 @@ int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     return rc;
+}
+
+int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
+    return ice_stream_bench(device, bytes, reps, 0, ms_per_copy);
 }
 
 int ice_get_info(IceHandle *h, IceInfo *info) {

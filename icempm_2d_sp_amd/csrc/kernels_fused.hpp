@@ -15,9 +15,10 @@
 // cell).  Lanes take the round's points one per lane in sorted order
 // (coalesced SoA loads and stores), run G2P, and stage the point's P2G terms
 // in LDS; then each cell lane sums the terms of its own points into a
-// register stencil and the workgroup adds the stencils in nine barrier-
-// separated phases of plain LDS read-add-write (within a phase all cells hit
-// distinct nodes).  Points that changed cell since the sort add their
+// register stencil, writes it back to LDS, and each node the round touches
+// gathers the stencils of its (up to) nine cells in a fixed order: plain LDS
+// adds, four barriers per batch of staged points.  Points that changed cell
+// since the sort add their
 // stencils with LDS atomics.  Points that change cell a second time leave
 // the tile pass (kEscaped + kFresh) and the off-tile kernel does their P2G,
 // with global atomics into the frame of the tile that now holds them.
@@ -39,27 +40,32 @@ __device__ __forceinline__ size_t frame_offset(const DomainDesc &dd, int tx, int
 }
 
 // Sum of node (x, y) of a domain over every frame that covers it, ascending tx
-// then ty.  (x, y) must lie inside the domain's grid.
+// then ty.  (x, y) must lie inside the domain's grid.  Branch-free: the four
+// candidate frames are always read (a covered-or-not flag selects the value,
+// uncovered candidates read the frame of the tile holding the node), so the
+// loads of several nodes can be in flight together.
 template <int TW, int TH>
 __device__ __forceinline__ void frame_node_sum(const float *__restrict__ F, const DomainDesc &dd, int x, int y,
                                                float &m, float &px, float &py) {
     using Fr = Frame<TW, TH>;
     m = px = py = 0.f;
     const int ta = (x + 2) / TW, ua = (y + 2) / TH;
+    const size_t home = frame_offset<TW, TH>(dd, x / TW, y / TH) + (x % TW + 2) * Fr::H + (y % TH + 2);
 #pragma unroll
     for (int dx = 1; dx >= 0; dx--) {
         const int tx = ta - dx;
         const int lx = x - tx * TW + 2;
-        if (tx < 0 || tx >= dd.ntx || lx >= Fr::W) continue;
+        const bool okx = tx >= 0 && tx < dd.ntx && lx < Fr::W;
 #pragma unroll
         for (int dy = 1; dy >= 0; dy--) {
             const int ty = ua - dy;
             const int ly = y - ty * TH + 2;
-            if (ty < 0 || ty >= dd.nty || ly >= Fr::H) continue;
-            const float *f = F + frame_offset<TW, TH>(dd, tx, ty) + lx * Fr::H + ly;
-            m += f[0];
-            px += f[Fr::N];
-            py += f[2 * Fr::N];
+            const bool ok = okx && ty >= 0 && ty < dd.nty && ly < Fr::H;
+            const size_t o = ok ? frame_offset<TW, TH>(dd, tx, ty) + lx * Fr::H + ly : home;
+            const float a = F[o], b = F[o + Fr::N], c = F[o + 2 * Fr::N];
+            m += ok ? a : 0.f;
+            px += ok ? b : 0.f;
+            py += ok ? c : 0.f;
         }
     }
 }
@@ -77,9 +83,10 @@ __device__ __forceinline__ void frame_node_velocity(const float *__restrict__ F,
     node_velocity(m, mx, my, G.status[node], cx, cy, G.wind_x[node], G.wind_y[node], ramp, P, vx, vy, wall);
 }
 
-// P2G terms of a point from its state in registers (kernels.cuh:54-76).
+// P2G terms of a point from its state in registers (kernels.cuh:54-76); v
+// holds the invariants of the point's Fe.
 __device__ __forceinline__ void p2g_terms(const DevParams &P, float x, float y, int cx, int cy, float vx, float vy,
-                                          const M2 &Fe, const M2 &Bp, float Jp, float th, P2GPoint &q) {
+                                          const FInv &v, const M2 &Bp, float Jp, float th, P2GPoint &q) {
     q.x = x;
     q.y = y;
     q.cx = cx;
@@ -87,7 +94,6 @@ __device__ __forceinline__ void p2g_terms(const DevParams &P, float x, float y, 
     q.mass = P.particle_mass * th;
     q.mvx = q.mass * vx;
     q.mvy = q.mass * vy;
-    const FInv v = invariants(Fe);
     float t00, t01, t11;
     kirchhoff(v, P, t00, t01, t11);
     const float k = P.dt_vol_Dpinv * Jp * th;
@@ -148,8 +154,54 @@ __device__ __forceinline__ void frame_scatter_global(const P2GPoint &q, const Do
         }
 }
 
-constexpr int kStage = 1024;   // staged points per sub-round
+// Points per lane whose loads are in flight together (one batch of
+// kPasses * 256 points is loaded, then computed), and staged per sub-round.
+#ifndef ICE_PIPE_PASSES
+#define ICE_PIPE_PASSES 4
+#endif
+constexpr int kPasses = ICE_PIPE_PASSES;
+constexpr int kStage = kPasses * kTileThreads;
 constexpr int kStageFields = 9;
+// After a batch is summed, the staging area holds the 27 stencil sums of each
+// of the round's 256 cells.
+static_assert(kStageFields * kStage >= 27 * kTileThreads, "staging area too small for the cell stencils");
+
+// What G2P reads of a point, loaded ahead of the computation.
+struct G2PRaw {
+    uint32_t u, cell;
+    float x, y, f00, f01, f10, f11, jp, th, dpt;
+};
+
+__device__ __forceinline__ void load_g2p_raw(const Lines &L, const DevParams &P, uint32_t i, G2PRaw &r) {
+    r.u = L.u(ICE_IDX_UTILITY, i);
+    r.cell = L.u(ICE_IDX_CELL, i);
+    r.x = L(ICE_IDX_POSX, i);
+    r.y = L(ICE_IDX_POSX + 1, i);
+    r.f00 = L(ICE_IDX_FE00, i);
+    r.f01 = L(ICE_IDX_FE00 + 1, i);
+    r.f10 = L(ICE_IDX_FE00 + 2, i);
+    r.f11 = L(ICE_IDX_FE00 + 3, i);
+    r.jp = L(ICE_IDX_JP_INV, i);
+    r.th = L(ICE_IDX_THICKNESS, i);
+    r.dpt = dp_threshold(P, L, i);
+}
+
+// A point's P2G terms into staging slot jj (mass 0 marks an empty slot).
+__device__ __forceinline__ void stage_terms(float (*s_st)[kStage], uint32_t jj, bool stage, const P2GPoint &q) {
+    if (stage) {
+        s_st[0][jj] = q.x;
+        s_st[1][jj] = q.y;
+        s_st[2][jj] = q.mass;
+        s_st[3][jj] = q.mvx;
+        s_st[4][jj] = q.mvy;
+        s_st[5][jj] = q.s00;
+        s_st[6][jj] = q.s01;
+        s_st[7][jj] = q.s10;
+        s_st[8][jj] = q.s11;
+    } else {
+        s_st[2][jj] = 0.f;
+    }
+}
 
 // FUSED = false: P2G of the current point state into frames Fn (fills the
 // pipeline).  FUSED = true: update + G2P of substep S from frames Fc, then P2G
@@ -194,21 +246,43 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
     if constexpr (FUSED) {
         resolve_step(S, P, time, rec);
         ramp = current_ramp(time);
-        for (int k = threadIdx.x; k < Fr::N; k += kTileThreads) {
+        // node velocities of T +- 2: every node's loads first (coordinates
+        // clamped into the grid, so all loads are unconditional), then the update
+        constexpr int NIT = (Fr::N + kTileThreads - 1) / kTileThreads;
+        float nm[NIT], nmx[NIT], nmy[NIT], nwx[NIT], nwy[NIT], ncx[NIT], ncy[NIT];
+        uint8_t nst[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int k = min(it * kTileThreads + (int)threadIdx.x, Fr::N - 1);
             const int lx = k / Fr::H, ly = k - (k / Fr::H) * Fr::H;
-            const int x = fx0 + lx, y = fy0 + ly;
-            float2 v = make_float2(0.f, 0.f);
-            if (x >= 0 && x < P.gx && y >= 0 && y < P.gy) {
-                float m, mx, my;
-                bool wall;
-                frame_node_velocity<TW, TH>(Fc, dd, G, P, ramp, x, y, v.x, v.y, mx, my, wall, m);
-                if (lx >= 2 && lx < TW + 2 && ly >= 2 && ly < TH + 2) {   // T's own node
-                    const size_t node = (size_t)x * P.gy + y;
-                    if (wall) { G.fx[node] += mx; G.fy[node] += my; }
-                    if (m != 0.f && (__builtin_isnan(v.x) || __builtin_isnan(v.y))) err |= ICE_ERR_GRID_NAN;
+            const int x = min(max(fx0 + lx, 0), P.gx - 1), y = min(max(fy0 + ly, 0), P.gy - 1);
+            frame_node_sum<TW, TH>(Fc, dd, x, y, nm[it], nmx[it], nmy[it]);
+            const size_t node = (size_t)x * P.gy + y;
+            nst[it] = G.status[node];
+            nwx[it] = G.wind_x[node];
+            nwy[it] = G.wind_y[node];
+            ncx[it] = P.has_current ? G.cur_x[node] : 0.f;
+            ncy[it] = P.has_current ? G.cur_y[node] : 0.f;
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int k = it * kTileThreads + (int)threadIdx.x;
+            if (k < Fr::N) {
+                const int lx = k / Fr::H, ly = k - (k / Fr::H) * Fr::H;
+                const int x = fx0 + lx, y = fy0 + ly;
+                float2 v = make_float2(0.f, 0.f);
+                if (x >= 0 && x < P.gx && y >= 0 && y < P.gy) {
+                    bool wall;
+                    node_velocity(nm[it], nmx[it], nmy[it], nst[it], ncx[it], ncy[it], nwx[it], nwy[it], ramp, P, v.x,
+                                  v.y, wall);
+                    if (lx >= 2 && lx < TW + 2 && ly >= 2 && ly < TH + 2) {   // T's own node
+                        const size_t node = (size_t)x * P.gy + y;
+                        if (wall) { G.fx[node] += nmx[it]; G.fy[node] += nmy[it]; }
+                        if (nm[it] != 0.f && (__builtin_isnan(v.x) || __builtin_isnan(v.y))) err |= ICE_ERR_GRID_NAN;
+                    }
                 }
+                s_v[k] = v;
             }
-            s_v[k] = v;
         }
     }
     for (int k = threadIdx.x; k < 3 * Fr::NN; k += kTileThreads) s_acc[k] = 0.f;
@@ -230,74 +304,100 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
             flen = cs[cell + 1] - b;
         }
         for (uint32_t sub = 0; sub < n_round; sub += kStage) {   // uniform bound
-#pragma unroll 1
-            for (int p = 0; p < kStage / kTileThreads; p++) {
+            // locate this lane's point of each pass (slot, or ~0u for none)
+            uint32_t slot[kPasses];
+#pragma unroll
+            for (int p = 0; p < kPasses; p++) {
+                slot[p] = ~0u;
                 const uint32_t jw = sub + (uint32_t)(p * kTileThreads + wave * 64);
-                const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;   // staging slot
-                if (jw >= n_round) {   // the whole wave is past the round's points
-                    s_st[2][jj] = 0.f;
-                    continue;
+                if (jw < n_round) {   // wave-uniform
+                    const int cb = flat_col_n(jw, s_pre, ncol);
+                    const uint32_t j = jw + lane;
+                    if (j < n_round) {
+                        const int c = col_from_n(j, cb, s_pre, ncol);
+                        slot[p] = s_rb[c] + (j - s_pre[c]);
+                    }
                 }
-                const int cb = flat_col_n(jw, s_pre, ncol);
-                const uint32_t j = jw + lane;
-                bool stage = false;
-                P2GPoint q;
-                if (j < n_round) {
-                    const int c = col_from_n(j, cb, s_pre, ncol);
-                    const uint32_t i = s_rb[c] + (j - s_pre[c]);
-                    const uint32_t u = L.u(ICE_IDX_UTILITY, i);
-                    if constexpr (FUSED) {
+            }
+            if constexpr (FUSED) {
+                // every pass's loads first, then the computation
+                G2PRaw raw[kPasses];
+#pragma unroll
+                for (int p = 0; p < kPasses; p++)
+                    if (slot[p] != ~0u) load_g2p_raw(L, P, slot[p], raw[p]);
+#pragma unroll
+                for (int p = 0; p < kPasses; p++) {
+                    const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;   // staging slot
+                    bool stage = false;
+                    P2GPoint q;
+                    const uint32_t i = slot[p];
+                    if (i != ~0u && !(raw[p].u & (kDisabled | kEscaped))) {
                         G2PState s;
-                        load_g2p_state(L, i, u, s);
-                        const float th = L(ICE_IDX_THICKNESS, i);
-                        const float dpt = dp_threshold(P, L, i);
-                        if (!(u & (kDisabled | kEscaped))) {
-                            const int rx = s.cx - fx0, ry = s.cy - fy0;
-                            if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2) {
-                                err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
-                            } else {
-                                float wx[3], wy[3];
-                                weights(s.x, wx);
-                                weights(s.y, wy);
-                                float pvx = 0.f, pvy = 0.f;
-                                M2 C{0.f, 0.f, 0.f, 0.f};
+                        s.u = raw[p].u;
+                        s.cx = (int)(raw[p].cell & 0xffff);
+                        s.cy = (int)(raw[p].cell >> 16);
+                        s.x = raw[p].x;
+                        s.y = raw[p].y;
+                        s.Fe = M2{raw[p].f00, raw[p].f01, raw[p].f10, raw[p].f11};
+                        s.Jp = raw[p].jp;
+                        const float th = raw[p].th;
+                        const int rx = s.cx - fx0, ry = s.cy - fy0;
+                        if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2) {
+                            err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
+                        } else {
+                            float wx[3], wy[3];
+                            weights(s.x, wx);
+                            weights(s.y, wy);
+                            float pvx = 0.f, pvy = 0.f;
+                            M2 C{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                                for (int a = -1; a <= 1; a++)
+                            for (int a = -1; a <= 1; a++)
 #pragma unroll
-                                    for (int b = -1; b <= 1; b++) {
-                                        const float w = wx[a + 1] * wy[b + 1];
-                                        const float2 nv = s_v[(rx + a) * Fr::H + (ry + b)];
-                                        const float dx = (float)a - s.x, dy = (float)b - s.y;
-                                        pvx += w * nv.x;
-                                        pvy += w * nv.y;
-                                        const float f = P.four_h_inv * w;
-                                        C.a00 += f * (nv.x * dx);
-                                        C.a01 += f * (nv.x * dy);
-                                        C.a10 += f * (nv.y * dx);
-                                        C.a11 += f * (nv.y * dy);
-                                    }
-                                g2p_finish(P, s, pvx, pvy, C, th, dpt, err);
-                                if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
-                                bool escaped = false;
-                                if (s.cell_updated && !(s.u & kDisabled)) {
-                                    if (s.u & kMoved) {
-                                        s.u |= kEscaped | kFresh;
-                                        const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
-                                        offtile[e] = i;
-                                        escaped = true;
-                                    } else {
-                                        s.u |= kMoved;
-                                    }
+                                for (int b = -1; b <= 1; b++) {
+                                    const float w = wx[a + 1] * wy[b + 1];
+                                    const float2 nv = s_v[(rx + a) * Fr::H + (ry + b)];
+                                    const float dx = (float)a - s.x, dy = (float)b - s.y;
+                                    pvx += w * nv.x;
+                                    pvy += w * nv.y;
+                                    const float f = P.four_h_inv * w;
+                                    C.a00 += f * (nv.x * dx);
+                                    C.a01 += f * (nv.x * dy);
+                                    C.a10 += f * (nv.y * dx);
+                                    C.a11 += f * (nv.y * dy);
                                 }
-                                g2p_store(L, i, s, pvx, pvy, C, rec);
-                                if (!(s.u & kDisabled) && !escaped) {
-                                    p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, s.Fe, C, s.Jp, th, q);
-                                    if (s.u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
-                                    else stage = true;
+                            FInv vf;
+                            g2p_finish(P, s, pvx, pvy, C, th, raw[p].dpt, err, vf);
+                            if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
+                            bool escaped = false;
+                            if (s.cell_updated && !(s.u & kDisabled)) {
+                                if (s.u & kMoved) {
+                                    s.u |= kEscaped | kFresh;
+                                    const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
+                                    offtile[e] = i;
+                                    escaped = true;
+                                } else {
+                                    s.u |= kMoved;
                                 }
                             }
+                            g2p_store(L, i, s, pvx, pvy, C, rec);
+                            if (!(s.u & kDisabled) && !escaped) {
+                                p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);
+                                if (s.u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
+                                else stage = true;
+                            }
                         }
-                    } else {
+                    }
+                    stage_terms(s_st, jj, stage, q);
+                }
+            } else {
+#pragma unroll
+                for (int p = 0; p < kPasses; p++) {
+                    const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;
+                    bool stage = false;
+                    P2GPoint q;
+                    const uint32_t i = slot[p];
+                    if (i != ~0u) {
+                        const uint32_t u = L.u(ICE_IDX_UTILITY, i);
                         if (!(u & (kDisabled | kEscaped))) {
                             load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
                             if (!stencil_inside(q.cx, q.cy, P)) err |= ICE_ERR_POINT_LEFT_AREA;
@@ -305,19 +405,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                             else stage = true;
                         }
                     }
-                }
-                if (stage) {
-                    s_st[0][jj] = q.x;
-                    s_st[1][jj] = q.y;
-                    s_st[2][jj] = q.mass;
-                    s_st[3][jj] = q.mvx;
-                    s_st[4][jj] = q.mvy;
-                    s_st[5][jj] = q.s00;
-                    s_st[6][jj] = q.s01;
-                    s_st[7][jj] = q.s10;
-                    s_st[8][jj] = q.s11;
-                } else {
-                    s_st[2][jj] = 0.f;   // no staged contribution
+                    stage_terms(s_st, jj, stage, q);
                 }
             }
             __syncthreads();
@@ -366,20 +454,44 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                         if (__builtin_isnan(am[a][b])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
                     }
             }
-            // nine phases, one per stencil offset: distinct nodes within a phase
-            const int lxc = cx - fx0, lyc = cy - fy0;
+            __syncthreads();   // staged terms consumed: the area now takes the cell stencils
+            float *s_cs = &s_st[0][0];   // [27][kTileThreads]: component-major, offset (a, b), cell lane
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
-                    if (any) {
-                        const int o = (lxc + a - 1) * Fr::LP + (lyc + b - 1);
-                        s_acc[o] += am[a][b];
-                        s_acc[Fr::NN + o] += ax[a][b];
-                        s_acc[2 * Fr::NN + o] += ay[a][b];
-                    }
-                    __syncthreads();
+                    const int q = a * 3 + b;
+                    s_cs[q * kTileThreads + threadIdx.x] = am[a][b];
+                    s_cs[(9 + q) * kTileThreads + threadIdx.x] = ax[a][b];
+                    s_cs[(18 + q) * kTileThreads + threadIdx.x] = ay[a][b];
                 }
+            __syncthreads();
+            // each node the round touches gathers its (up to) nine cell stencils
+            // in a fixed order: frame columns col0 + 1 .. col0 + ncol + 2, rows 1 .. TH + 2
+            const int nyn = TH + 2;
+            for (int k = threadIdx.x; k < (ncol + 2) * nyn; k += kTileThreads) {
+                const int ix = k / nyn, iy = k - ix * nyn;
+                float sm = 0.f, sx = 0.f, sy = 0.f;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const int ccn = ix - a;
+                    if (ccn < 0 || ccn >= ncol) continue;
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        const int rrn = iy - b;
+                        if (rrn < 0 || rrn >= TH) continue;
+                        const int c = ccn * TH + rrn, q = a * 3 + b;
+                        sm += s_cs[q * kTileThreads + c];
+                        sx += s_cs[(9 + q) * kTileThreads + c];
+                        sy += s_cs[(18 + q) * kTileThreads + c];
+                    }
+                }
+                const int o = (col0 + 1 + ix) * Fr::LP + (1 + iy);
+                s_acc[o] += sm;
+                s_acc[Fr::NN + o] += sx;
+                s_acc[2 * Fr::NN + o] += sy;
+            }
+            __syncthreads();   // stencils read: the area takes the next staged terms
         }
         __syncthreads();   // the round's ranges are read; the next round may overwrite them
     }
@@ -455,7 +567,8 @@ __global__ __launch_bounds__(256) void k_offtile_pipeline(DevParams P0, StepArg 
                     C.a10 += f * (nvy * dx);
                     C.a11 += f * (nvy * dy);
                 }
-            g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err);
+            FInv vf;
+            g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
             if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
             g2p_store(L, i, s, pvx, pvy, C, rec);
             if (s.u & kDisabled) continue;

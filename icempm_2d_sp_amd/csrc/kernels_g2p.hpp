@@ -98,7 +98,8 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiles(DevParams P0, StepAr
                 C.a10 += f * (nv.y * dx);
                 C.a11 += f * (nv.y * dy);
             }
-        g2p_finish(P, s, pvx, pvy, C, th, dpt, err);
+        FInv vf;
+        g2p_finish(P, s, pvx, pvy, C, th, dpt, err, vf);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
         if (s.cell_updated && !(s.u & kDisabled)) {
             if (s.u & kMoved) {
@@ -174,7 +175,8 @@ __global__ __launch_bounds__(256) void k_g2p_offtile(DevParams P0, StepArg S, co
                 C.a10 += f * (nvy * dx);
                 C.a11 += f * (nvy * dy);
             }
-        g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err);
+        FInv vf;
+        g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
         g2p_store(L, i, s, pvx, pvy, C, rec);
     }

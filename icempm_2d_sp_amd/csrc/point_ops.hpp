@@ -35,16 +35,17 @@ struct Counters {
     uint32_t pad[4];
 };
 
+// The SoA lines of a point range.  Slot i of a line is addressed as the
+// line's (wave-uniform) base plus a 32-bit byte offset, so every access of a
+// lane shares one offset register (the library keeps pitch below 2^30).
 struct Lines {
     float *base;
     size_t pitch;
-    __device__ __forceinline__ float &operator()(int line, uint32_t i) const { return base[(size_t)line * pitch + i]; }
-    __device__ __forceinline__ uint32_t u(int line, uint32_t i) const {
-        return __float_as_uint(base[(size_t)line * pitch + i]);
+    __device__ __forceinline__ float &operator()(int line, uint32_t i) const {
+        return *reinterpret_cast<float *>(reinterpret_cast<char *>(base + (size_t)line * pitch) + (uint32_t)(i << 2));
     }
-    __device__ __forceinline__ void set_u(int line, uint32_t i, uint32_t v) const {
-        base[(size_t)line * pitch + i] = __uint_as_float(v);
-    }
+    __device__ __forceinline__ uint32_t u(int line, uint32_t i) const { return __float_as_uint((*this)(line, i)); }
+    __device__ __forceinline__ void set_u(int line, uint32_t i, uint32_t v) const { (*this)(line, i) = __uint_as_float(v); }
 };
 
 // P2G inputs of one point: mass and the APIC affine momentum terms
@@ -104,8 +105,9 @@ struct G2PState {
     bool newly_disabled;
 };
 
+// vf receives the invariants of the final Fe (for the next P2G's stress).
 __device__ __forceinline__ void g2p_finish(const DevParams &P, G2PState &s, float pvx, float pvy, const M2 &C,
-                                           float thickness, float dpt, uint32_t &err) {
+                                           float thickness, float dpt, uint32_t &err, FInv &vf) {
     s.x += pvx * P.dt_h_inv;
     s.y += pvy * P.dt_h_inv;
     if (__builtin_isnan(pvx) || __builtin_isnan(pvy)) err |= ICE_ERR_POINT_VEL_NAN;
@@ -130,11 +132,11 @@ __device__ __forceinline__ void g2p_finish(const DevParams &P, G2PState &s, floa
     s.Fe.a10 = F.a10 + (c10 * F.a00 + c11 * F.a10);
     s.Fe.a11 = F.a11 + (c10 * F.a01 + c11 * F.a11);
     if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
-    const FInv v = invariants(s.Fe);
-    compute_pq(v, P, s.p_tr, s.q_tr);
+    vf = invariants(s.Fe);
+    compute_pq(vf, P, s.p_tr, s.q_tr);
     if (!(s.u & kCrushed) && outside_failure_surface(s.p_tr, s.q_tr, P)) s.u |= kCrushed;
     if (s.u & kCrushed) {
-        return_map(v, s.p_tr, s.q_tr, thickness, dpt, P, s.Fe, s.Jp);
+        if (return_map(vf, s.p_tr, s.q_tr, thickness, dpt, P, s.Fe, s.Jp) != 2) vf = invariants(s.Fe);
         if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
     }
 }

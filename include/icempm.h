@@ -113,7 +113,8 @@ int ice_destroy(IceHandle *h);
 const char *ice_last_error(const IceHandle *h);
 
 /* GPU_Partition::allocate (gpu_partition.cpp:259-293).  Device pitch is
- * reported by ice_get_info. */
+ * reported by ice_get_info.  A handle holds fewer than 2^30 point slots
+ * (ICE_E_CAPACITY beyond). */
 int ice_allocate(IceHandle *h, uint32_t n_points_capacity);
 
 /* GPU_Partition::transfer_points_from_soa_to_device (gpu_partition.cpp:81-106):
@@ -236,6 +237,9 @@ int ice_get_domain_info(IceHandle *h, int d, IceInfo *info);
  * device-to-device copy of `bytes` (read + write), over `reps` launches on
  * `device`.  Used to read HBM bandwidth and to scale profiler byte counters. */
 int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy);
+/* The same for mode 0 (copy: read + write `bytes`), 1 (read `bytes`) or 2
+ * (write `bytes`). */
+int ice_stream_bench(int device, size_t bytes, int reps, int mode, double *ms_per_pass);
 
 #ifdef __cplusplus
 }
