@@ -29,7 +29,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n_points: int, n_nodes: int, cohesion: bool) -> dict:
+def algorithmic_bytes(n_points: int, n_nodes: int, cohesion: bool, complete_frac: float = 1.0,
+                      rec_frac: float = 0.0) -> dict:
     """Bytes each kernel must move once per substep (FP32 hssoa, 4 B slots).
 
     Unfused: P2G reads utility, cell, pos x2, vel x2, Fe x4, Bp x4, Jp_inv,
@@ -39,13 +40,15 @@ def algorithmic_bytes(n_points: int, n_nodes: int, cohesion: bool) -> dict:
     per node it reads m, px, py, the status byte and the 2-float wind stress.
 
     Pipelined (one tile pass per substep): per point the same 10 (+1) slots
-    read and 14 written (the cell word only changes on a crossing); per node
-    the node sums read and written once (24 B), status (1 B), wind (8 B) and
-    the cell start (4 B).
+    read; position and Fe (6 slots) written every substep, velocity and Bp
+    (6 slots) on the substeps that complete a run chunk (complete_frac of
+    them), P and Q on recording substeps (rec_frac); Jp_inv, the utility and
+    cell words only when they change (not counted).  Per node the node sums
+    read and written once (24 B), status (1 B), wind (8 B), cell start (4 B).
     """
     p2g = 64 * n_points + 12 * n_nodes
     g2p = (40 + (4 if cohesion else 0) + 60) * n_points + 21 * n_nodes
-    pipe = (40 + (4 if cohesion else 0) + 56) * n_points + 37 * n_nodes
+    pipe = (40 + (4 if cohesion else 0) + 24 + 24 * complete_frac + 8 * rec_frac) * n_points + 37 * n_nodes
     return {"p2g": p2g, "g2p": g2p, "substep": p2g + g2p, "pipelined": pipe}
 
 
@@ -140,7 +143,13 @@ def main() -> int:
     ms_g2p = tm[3] / nsub
     ms_sub_ev = tm[4] / nsub
     nodes = gx * gy
-    B = algorithmic_bytes(n_active, nodes, True)
+    # substeps of the timed run that store the complete point state (the last
+    # of each 16-substep chunk) and that record P, Q
+    chunks = [min(16, args.steps - k) for k in range(0, args.steps, 16)]
+    every = scene.params.UpdateEveryNthStep
+    n_rec = sum(1 for k in range(args.warmup, args.warmup + args.steps) if every > 0 and k % every == 0)
+    B = algorithmic_bytes(n_active, nodes, True, complete_frac=len(chunks) / args.steps,
+                          rec_frac=n_rec / args.steps)
     if pipelined:
         dominant, ms_dom, kname = "pipelined", ms_g2p, "k_tile_pipeline"
         B["substep"] = B["pipelined"]

@@ -646,6 +646,7 @@ int build_graph(IceHandle *h, int chunk) {
     int rc = ICE_OK;
     for (int k = 0; k < chunk && rc == ICE_OK; k++) {
         StepArg S{0.0, 0, &h->step_cnt()->step, k};
+        S.complete = k == chunk - 1;
         rc = substep_args(h, S);
     }
     if (rc == ICE_OK) hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, h->stream, h->step_cnt(), chunk);
@@ -1155,6 +1156,7 @@ int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
         } else {
             for (int k = 0; k < chunk; k++) {
                 StepArg S{0.0, 0, &h->step_cnt()->step, k};
+                S.complete = k == chunk - 1;
                 if ((rc = substep_args(h, S))) return rc;
             }
             hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, h->stream, h->step_cnt(), chunk);

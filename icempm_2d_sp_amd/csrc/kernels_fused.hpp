@@ -379,7 +379,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                                     s.u |= kMoved;
                                 }
                             }
-                            g2p_store(L, i, s, pvx, pvy, C, rec);
+                            // escaped and disabled points leave the pipeline: store all
+                            g2p_store_pipelined(L, i, s, raw[p].u, raw[p].jp, pvx, pvy, C, rec,
+                                                S.complete || escaped || (s.u & kDisabled));
                             if (!(s.u & kDisabled) && !escaped) {
                                 p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);
                                 if (s.u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);

@@ -13,6 +13,10 @@ struct StepArg {
     int recordPQ;
     const int *step_ptr;   // when set: step = *step_ptr + step_off
     int step_off;
+    // Pipelined path only: 0 lets the substep skip storing what no later
+    // substep reads (velocity and Bp, which the next P2G takes from registers);
+    // the last substep before control returns to the caller stores everything.
+    int complete = 1;
 };
 
 __device__ __forceinline__ void resolve_step(const StepArg &s, const DevParams &P, double &t, bool &rec) {
@@ -164,6 +168,35 @@ __device__ __forceinline__ void g2p_store(const Lines &L, uint32_t i, const G2PS
     L(ICE_IDX_JP_INV, i) = s.Jp;
     L.set_u(ICE_IDX_UTILITY, i, s.u);
     if (s.cell_updated) L.set_u(ICE_IDX_CELL, i, ((uint32_t)s.cy << 16) | (uint32_t)s.cx);
+    if (rec) {
+        L(ICE_IDX_P, i) = s.p_tr;
+        L(ICE_IDX_Q, i) = s.q_tr;
+    }
+}
+
+// The pipelined path's store: position and Fe every substep; Jp_inv, the
+// utility word and the cell word when they changed; velocity and Bp (read by
+// no later substep of the pipeline) only when `complete`; P and Q when rec.
+__device__ __forceinline__ void g2p_store_pipelined(const Lines &L, uint32_t i, const G2PState &s, uint32_t u0,
+                                                   float jp0, float pvx, float pvy, const M2 &C, bool rec,
+                                                   bool complete) {
+    L(ICE_IDX_POSX, i) = s.x;
+    L(ICE_IDX_POSX + 1, i) = s.y;
+    L(ICE_IDX_FE00, i) = s.Fe.a00;
+    L(ICE_IDX_FE00 + 1, i) = s.Fe.a01;
+    L(ICE_IDX_FE00 + 2, i) = s.Fe.a10;
+    L(ICE_IDX_FE00 + 3, i) = s.Fe.a11;
+    if (__float_as_uint(s.Jp) != __float_as_uint(jp0)) L(ICE_IDX_JP_INV, i) = s.Jp;
+    if (s.u != u0) L.set_u(ICE_IDX_UTILITY, i, s.u);
+    if (s.cell_updated) L.set_u(ICE_IDX_CELL, i, ((uint32_t)s.cy << 16) | (uint32_t)s.cx);
+    if (complete) {
+        L(ICE_IDX_VELX, i) = pvx;
+        L(ICE_IDX_VELX + 1, i) = pvy;
+        L(ICE_IDX_BP00, i) = C.a00;
+        L(ICE_IDX_BP00 + 1, i) = C.a01;
+        L(ICE_IDX_BP00 + 2, i) = C.a10;
+        L(ICE_IDX_BP00 + 3, i) = C.a11;
+    }
     if (rec) {
         L(ICE_IDX_P, i) = s.p_tr;
         L(ICE_IDX_Q, i) = s.q_tr;

@@ -76,7 +76,10 @@ def test_pipelined_run_with_resorts_matches_oracle():
     o = oracle_run(sc, 96)
     _print("sorts", info.n_sorts - n0, "offtile", info.n_offtile, "disabled", dis, o.disabled)
     assert info.n_sorts - n0 >= 1
-    rep = compare_points(pts, o.points(), fields=MANY, max_bad_frac=0.01, max_flag_mismatch=0.01)
+    # inside a run chunk the pipeline skips storing velocity and Bp; the
+    # chunk's last substep stores them, so they are checked here too
+    rep = compare_points(pts, o.points(), fields=MANY + [("Bp", list(range(abi.IDX_BP00, abi.IDX_BP00 + 4)), 1e-3)],
+                         max_bad_frac=0.01, max_flag_mismatch=0.01)
     _print(rep)
     assert dis == o.disabled
     assert err == o.error
