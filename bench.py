@@ -7,11 +7,14 @@ Gaussian-random-field wind stress, lognormal per-point cohesion, dt = 1e-4 s.
 Inputs are resident in HBM when the timed region starts.  A "step" is one
 substep (P2G, node update, G2P with SVD and return map) over all points.
 
-The timed region runs K substeps with HIP events around each kernel on the
-library's stream; `value` = active points * K / wall time.  `roofline`
-prices the dominant kernel's algorithmic bytes per launch against MI355X's
-8 TB/s.  `cpu_baseline` times the FP64 oracle on the host cores over a
-bounded sample of the same workload.
+The timed region replays K substeps from HIP graphs (16 substeps per graph,
+as ice_run does), bracketed by stream synchronisation; `value` = active
+points * K / wall time.  A second, shorter pass of the same substeps runs
+eagerly with HIP events around each kernel on the library's stream; it gives
+the dominant kernel's average launch time, from which `roofline` prices the
+kernel's algorithmic bytes per launch against MI355X's 8 TB/s.
+`cpu_baseline` times the FP64 oracle on the host cores over a bounded sample
+of the same workload.
 
 Prints ONE JSON line.
 """
@@ -52,6 +55,20 @@ def algorithmic_bytes(n_points: int, n_nodes: int, cohesion: bool, complete_frac
     return {"p2g": p2g, "g2p": g2p, "substep": p2g + g2p, "pipelined": pipe}
 
 
+def measured_traffic(kernel: str, tile: str, grid: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary for this workload and tile (FETCH_SIZE and WRITE_SIZE passes,
+    scaled by the same counters over the plain copy kernel), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"{kernel}_icefield{grid}_*.md"))):
+        for line in open(path):
+            m = re.match(r"traffic_bytes_per_launch: tile (\S+) ([0-9.eE+]+)", line.strip())
+            if m and m.group(1) == tile:
+                return float(m.group(2))
+    return None
+
+
 def cpu_baseline(scene, target_s: float, threads: int, max_steps: int = 50) -> dict:
     """FP64 oracle on the host cores: 1 warm-up substep, then whole substeps
     until about target_s seconds of CPU work (at least 2)."""
@@ -65,24 +82,33 @@ def cpu_baseline(scene, target_s: float, threads: int, max_steps: int = 50) -> d
     steps += 1
     n = o.n
     o.close()
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": n * steps / secs, "unit": "particle-updates/s", "cores": nthr, "kind": "port",
-            "sample": f"FP64 oracle (OpenMP, {nthr} threads), {steps} substeps of the same {n}-point "
-                      f"workload after 1 warm-up substep, {secs:.1f} s"}
+            "sample": f"FP64 oracle (OpenMP, {nthr} threads; host nproc {os.cpu_count()}, {model}), {steps} "
+                      f"substeps of the same {n}-point workload after 1 warm-up substep, {secs:.1f} s"}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--tile", type=str, default="32x32")
+    ap.add_argument("--event-steps", type=int, default=48, help="eager substeps timed per kernel with HIP events")
+    ap.add_argument("--tile", type=str, default="16x32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--naive", action="store_true", help="reference launch structure (unsorted, atomics)")
     ap.add_argument("--unfused", action="store_true", help="separate P2G and update + G2P tile kernels")
     ap.add_argument("--scale", type=float, default=1.0, help="linear domain scale (1.0 = 4096x2048)")
-    ap.add_argument("--workload", choices=["config2", "batch"], default="config2",
-                    help="config2: one 4096x2048 domain (the headline); batch: config 3's 64 uneven domains")
+    ap.add_argument("--workload", choices=["icefield_4096x2048", "batch"], default="icefield_4096x2048",
+                    help="icefield: one 4096x2048 domain (the headline); batch64: 64 uneven domains")
     ap.add_argument("--copy-only", type=float, default=0.0,
                     help="only time a plain copy of this many GB (HBM reference / counter calibration)")
     args = ap.parse_args()
@@ -107,12 +133,12 @@ def main() -> int:
         print(json.dumps(out), flush=True)
         return 0
 
-    if args.workload == "batch":
+    if args.workload == "batch64":
         return bench_batch(args)
 
     gx, gy = int(round(4096 * args.scale)), int(round(2048 * args.scale))
     t0 = time.time()
-    scene = scenes.config2(gx=gx, gy=gy)
+    scene = scenes.icefield_4096x2048(gx=gx, gy=gy)
     t_gen = time.time() - t0
     print(f"[bench] scene {gx}x{gy}: {scene.n} points, generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
 
@@ -125,17 +151,23 @@ def main() -> int:
     sim.run(0, args.warmup, graph=False)
     sim.sync()
 
+    # timed region: graph replays of whole substeps
+    t0 = time.perf_counter()
+    sim.run(args.warmup, args.steps, graph=True)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    err, dis = _err_dis(sim)
+    print(f"[bench] timed {args.steps} substeps in {wall:.3f}s; err {err:#x}", file=sys.stderr, flush=True)
+
+    # per-kernel times: the next substeps, eagerly, with HIP events
+    ev_first, ev_steps = args.warmup + args.steps, args.event_steps
     sim.reset_timings()
     sim.enable_timings(True)
     sim.sync()
-    t0 = time.perf_counter()
-    sim.run(args.warmup, args.steps, graph=False)
+    sim.run(ev_first, ev_steps, graph=False)
     sim.sync()
-    wall = time.perf_counter() - t0
     tm = sim.timings()
     sim.enable_timings(False)
-    err, dis = _err_dis(sim)
-    print(f"[bench] timed {args.steps} substeps in {wall:.3f}s; err {err:#x}", file=sys.stderr, flush=True)
 
     nsub = max(tm[5], 1)
     ms_step = wall * 1000.0 / args.steps
@@ -143,13 +175,12 @@ def main() -> int:
     ms_g2p = tm[3] / nsub
     ms_sub_ev = tm[4] / nsub
     nodes = gx * gy
-    # substeps of the timed run that store the complete point state (the last
+    # substeps of the event pass that store the complete point state (the last
     # of each 16-substep chunk) and that record P, Q
-    chunks = [min(16, args.steps - k) for k in range(0, args.steps, 16)]
+    chunks = [min(16, ev_steps - k) for k in range(0, ev_steps, 16)]
     every = scene.params.UpdateEveryNthStep
-    n_rec = sum(1 for k in range(args.warmup, args.warmup + args.steps) if every > 0 and k % every == 0)
-    B = algorithmic_bytes(n_active, nodes, True, complete_frac=len(chunks) / args.steps,
-                          rec_frac=n_rec / args.steps)
+    n_rec = sum(1 for k in range(ev_first, ev_first + ev_steps) if every > 0 and k % every == 0)
+    B = algorithmic_bytes(n_active, nodes, True, complete_frac=len(chunks) / ev_steps, rec_frac=n_rec / ev_steps)
     if pipelined:
         dominant, ms_dom, kname = "pipelined", ms_g2p, "k_tile_pipeline"
         B["substep"] = B["pipelined"]
@@ -159,6 +190,9 @@ def main() -> int:
         dominant, ms_dom, kname = "p2g", ms_p2g, "k_p2g_tiles" if not args.naive else "k_p2g_naive"
     achieved = B[dominant] / (ms_dom * 1e-3) / 1e9
     value = n_active * args.steps / wall
+    # the two-pass byte count (P2G + update/G2P, 168 B/point + 33 B/node) per
+    # wall-clock substep: what the same throughput means for the unfused design
+    two_pass = (168 * n_active + 33 * nodes) / (ms_step * 1e-3) / 1e9
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -180,7 +214,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"config2: plain MPM, {gx}x{gy} nodes h=1 m, {n_active} points at 4/cell in a seeded "
+            "workload": (f"icefield_{gx}x{gy}: plain MPM, {gx}x{gy} nodes h=1 m, {n_active} points at 4/cell in a seeded "
                          "random polygonal ice field (~95%), GRF wind stress (mean 0.2 Pa, std 0.05 Pa, "
                          "corr 200 cells), lognormal cohesion (median 1e5 Pa, sigma 0.5), dt=1e-4 s"),
             "points": n_active, "nodes": nodes, "tile": args.tile,
@@ -190,12 +224,13 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "kernel": kname,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(kname, f"{tw}x{th}", f"{gx}x{gy}"),
             "bytes_per_launch": B[dominant], "ms_per_launch": ms_dom,
             "substep": {"bytes": B["substep"], "ms_events": ms_sub_ev,
                         "achieved_GBs": B["substep"] / (ms_sub_ev * 1e-3) / 1e9,
                         "frac": B["substep"] / (ms_sub_ev * 1e-3) / 1e9 / HBM_PEAK_GBS},
-            "p2g_ms": ms_p2g, "g2p_ms": ms_g2p,
+            "p2g_ms": ms_p2g, "g2p_ms": ms_g2p, "event_substeps": ev_steps,
+            "two_pass_equivalent": {"GBs": two_pass, "frac": two_pass / HBM_PEAK_GBS},
         },
         "cpu_baseline": cpu,
     }
@@ -204,14 +239,14 @@ def main() -> int:
 
 
 def bench_batch(args) -> int:
-    """Config 3: 64 independent domains (sides 128-1024 cells, 25-95 % ice) run
+    """batch64_uneven: 64 independent domains (sides 128-1024 cells, 25-95 % ice) run
     (a) as one batched handle (one launch per phase, HIP-graph replays, tile
     autotuned) and (b) one domain after another with the single-domain path,
     the reference's one-partition-at-a-time shape.  value = (a)."""
     from icempm_2d_sp_amd import scenes
     from icempm_2d_sp_amd.gpu import BatchSimulation, Simulation
     t0 = time.time()
-    doms = scenes.config3()
+    doms = scenes.batch64_uneven()
     print(f"[bench] {len(doms)} domains, {sum(d.n for d in doms)} points, generated in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     steps = max(16, (args.steps // 16) * 16)
@@ -244,7 +279,7 @@ def bench_batch(args) -> int:
         "unit": "particle-updates/s",
         "n_gpus": 1, "steps": steps, "warmup": 16, "ms_per_step": wall_b * 1000 / steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "config3: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, 4 points/cell, "
+        "config": {"workload": "batch64_uneven: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, 4 points/cell, "
                                "per-domain wind 0.05-0.3 Pa", "points": n_active, "tile": f"{tile[0]}x{tile[1]}"},
         "sequential_single_domain": {"value": n_active * steps / wall_s, "ms_per_step": wall_s * 1000 / steps},
         "batched_over_sequential": wall_s / wall_b,

@@ -118,7 +118,7 @@ def gaussian_random_field(gx: int, gy: int, corr_len: float, rng: np.random.Gene
     return f / s if s > 0 else f
 
 
-def config1(seed: int = 12345, gx: int = 256, gy: int = 128) -> Scene:
+def floe_256x128(seed: int = 12345, gx: int = 256, gy: int = 128) -> Scene:
     """Plain MPM, 256x128 nodes h = 1 m, ice in the left 60 % of the domain,
     4 points per cell, uniform wind stress 0.1 Pa along +x, still ocean,
     dt = 1e-4 s, walls on all sides."""
@@ -130,7 +130,7 @@ def config1(seed: int = 12345, gx: int = 256, gy: int = 128) -> Scene:
     prm.ParticleVolume = prm.cellsize ** 2 * gx * gy / n_gen
     wind = np.zeros((2, gx * gy), np.float32)
     wind[0] = 0.1
-    return Scene(prm, soa, border_status(gx, gy), None, wind, "config1")
+    return Scene(prm, soa, border_status(gx, gy), None, wind, "floe_256x128")
 
 
 def voronoi_ice_mask(ncx: int, ncy: int, n_floes: int, coverage: float, rng: np.random.Generator) -> np.ndarray:
@@ -148,7 +148,7 @@ def voronoi_ice_mask(ncx: int, ncy: int, n_floes: int, coverage: float, rng: np.
     return (gap > thr).reshape(ncx, ncy)
 
 
-def config2(seed: int = 2024, gx: int = 4096, gy: int = 2048, cohesion_cap: float = 1e6) -> Scene:
+def icefield_4096x2048(seed: int = 2024, gx: int = 4096, gy: int = 2048, cohesion_cap: float = 1e6) -> Scene:
     """Plain MPM at scale: 4096x2048 nodes, 4 points per cell in a seeded
     random polygonal ice field covering ~95 % of the domain, wind stress from
     a Gaussian random field (correlation length 200 cells, mean 0.2 Pa, std
@@ -169,7 +169,7 @@ def config2(seed: int = 2024, gx: int = 4096, gy: int = 2048, cohesion_cap: floa
     wind = np.empty((2, gx * gy), np.float32)
     wind[0] = (0.2 + 0.05 * gaussian_random_field(gx, gy, 200.0, rng)).reshape(-1)
     wind[1] = (0.05 * gaussian_random_field(gx, gy, 200.0, rng)).reshape(-1)
-    return Scene(prm, soa, border_status(gx, gy), None, wind, "config2")
+    return Scene(prm, soa, border_status(gx, gy), None, wind, "icefield_4096x2048")
 
 
 def dynamic_scene(seed: int = 7, gx: int = 48, gy: int = 40, *, speed: float = 300.0, strain: float = 4e-3,
@@ -238,7 +238,7 @@ def batch_domain(seed: int, side_cells: int, coverage: float, wind_pa: float, wi
     return Scene(prm, soa, border_status(gx, gy), None, wind, f"batch{seed}")
 
 
-def config3(seed: int = 3003, n_domains: int = 64, min_side: int = 128, max_side: int = 1024) -> list:
+def batch64_uneven(seed: int = 3003, n_domains: int = 64, min_side: int = 128, max_side: int = 1024) -> list:
     """Batched plain MPM: n_domains independent domains, sides log-uniform in
     [min_side, max_side] cells, ice coverage 25-95 % (so 16k-4M points at 4 per
     cell), per-domain seeded wind 0.05-0.3 Pa in a random direction."""

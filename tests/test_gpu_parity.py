@@ -129,19 +129,19 @@ def test_many_substeps_dynamic_with_crossings(pipelined):
     sim.close()
 
 
-def test_config1_200_substeps_graph():
-    """Config 1 (256x128, 60 % ice, 0.1 Pa wind) for 200 substeps through
+def test_floe_256x128_200_substeps_graph():
+    """The 256x128 floe (60 % ice, 0.1 Pa wind) for 200 substeps through
     ice_run with HIP-graph replays.
 
-    Config 1's strain increments (dt * C ~ 1e-10 per substep) are below the
+    The floe's strain increments (dt * C ~ 1e-10 per substep) are below the
     FP32 spacing of Fe near 1 (6e-8), so in FP32 the ice drifts rigidly while
     the FP64 oracle accumulates ~20 Pa of elastic stress near the sticky left
     wall.  The HIP path is therefore held to the FP32 oracle field by field,
     and to the FP64 oracle on what FP32 resolves: identical flags and cells,
     and the mean wind-driven drift velocity within 10 %.  Per-field FP64
     errors are printed."""
-    _print("test_config1_200_substeps_graph")
-    sc = scenes.config1()
+    _print("test_floe_256x128_200_substeps_graph")
+    sc = scenes.floe_256x128()
     sim = Simulation(sc)
     sim.run(0, 200, graph=True)
     pts, forces, err, dis = sim.download()
