@@ -63,6 +63,18 @@ class IceInfo(C.Structure):
     ]
 
 
+MLP_PARAMS = 4675
+
+
+class IceMlpNorm(C.Structure):
+    _fields_ = [("xm", C.c_float * 4), ("xs", C.c_float * 4), ("ym", C.c_float * 3), ("ys", C.c_float * 3)]
+
+
+class IceMlpSampling(C.Structure):
+    _fields_ = [("logJ_min", C.c_double), ("logJ_max", C.c_double), ("D_min", C.c_double), ("D_max", C.c_double),
+                ("jp_min", C.c_double), ("jp_max", C.c_double)]
+
+
 _P = C.c_void_p
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
@@ -98,6 +110,20 @@ SIGNATURES = {
     "ice_get_info": (C.c_int, [_P, C.POINTER(IceInfo)]),
     "ice_copy_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "ice_stream_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "ice_mlp_default_sampling": (C.c_int, [C.c_void_p]),
+    "ice_mlp_generate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_uint32)]),
+    "ice_mlp_set_norm": (C.c_int, [_P, C.c_void_p]),
+    "ice_mlp_get_norm": (C.c_int, [_P, C.c_void_p]),
+    "ice_mlp_set_params": (C.c_int, [_P, C.c_void_p]),
+    "ice_mlp_get_params": (C.c_int, [_P, C.c_void_p]),
+    "ice_mlp_init_params": (C.c_int, [_P, C.c_uint64]),
+    "ice_mlp_train": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.c_void_p]),
+    "ice_mlp_forward": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "ice_mlp_grad": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]),
+    "ice_mlp_eval": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]),
+    "ice_mlp_deploy": (C.c_int, [_P, C.c_int, C.c_float]),
+    "ice_mlp_counts": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "ice_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "ice_create_batch": (C.c_int, [C.POINTER(IceParams), C.c_int, C.c_int, _P, _P, _P, C.POINTER(_P)]),
     "ice_upload_domain": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.c_uint32, C.c_int, _P, _P, _P]),

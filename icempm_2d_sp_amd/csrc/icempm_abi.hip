@@ -111,6 +111,19 @@ struct IceHandle {
     hipGraphExec_t graph_exec = nullptr;
     int graph_chunk = 0;
     int graph_fr0 = 0;
+    // learned plastic projection
+    float *mlp_p = nullptr, *mlp_g = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_loss = nullptr;
+    bf16x4 *mlp_pk = nullptr;
+    uint32_t *mlp_fail = nullptr;
+    float *mlp_x = nullptr, *mlp_t = nullptr;   // generated pairs
+    uint32_t mlp_n = 0;
+    size_t mlp_loss_cap = 0;
+    MlpNorm mlp_norm{};
+    IceMlpSampling mlp_samp{};
+    bool mlp_have_norm = false, mlp_have_params = false;
+    uint32_t mlp_adam_t = 0;
+    bool mlp_deployed = false;
+    float mlp_tol = 1e-4f;
 
     int ndom() const { return (int)dom.size(); }
     float *gline(int l) const { return grid + (size_t)l * gpitch; }
@@ -254,8 +267,8 @@ using P2GFn = void (*)(DevParams, const DomainDesc *, const TileRef *, int, Line
                        Counters *, uint32_t *, FixScale);
 using G2PFn = void (*)(DevParams, StepArg, const DomainDesc *, const TileRef *, int, Lines, const uint32_t *,
                        GridLines, Counters *, uint32_t *);
-using PipeFn = void (*)(DevParams, StepArg, const DomainDesc *, const TileRef *, int, Lines, const uint32_t *,
-                        GridLines, Counters *, uint32_t *, const float *, float *);
+using PipeFn = void (*)(DevParams, StepArg, SurrArg, const DomainDesc *, const TileRef *, int, Lines,
+                        const uint32_t *, GridLines, Counters *, uint32_t *, const float *, float *);
 using OffFn = void (*)(DevParams, StepArg, const DomainDesc *, Lines, GridLines, Counters *, const uint32_t *,
                        const float *, float *, int);
 using ToGridFn = void (*)(const DomainDesc *, int, const float *, GridLines);
@@ -264,14 +277,15 @@ struct TileKernels {
     int w, h;
     P2GFn p2g, p2g_det;
     G2PFn g2p;
-    PipeFn fill, fused;
+    PipeFn fill, fused, fused_surr;
     OffFn offtile;
     ToGridFn to_grid;
 };
 
 #define TK(W, H)                                                                                          \
     {W, H, k_p2g_tiles<W, H, false>, k_p2g_tiles<W, H, true>, k_g2p_tiles<W, H>, k_tile_pipeline<W, H, false>, \
-     k_tile_pipeline<W, H, true>, k_offtile_pipeline<W, H>, k_frames_to_grid<W, H>}
+     k_tile_pipeline<W, H, true>, k_tile_pipeline<W, H, true, true>, k_offtile_pipeline<W, H>,               \
+     k_frames_to_grid<W, H>}
 const TileKernels kTiles[] = {TK(8, 8),   TK(8, 16),  TK(16, 8),  TK(16, 16), TK(16, 32),
                               TK(32, 16), TK(32, 32), TK(32, 64), TK(64, 32), TK(64, 64)};
 #undef TK
@@ -560,6 +574,20 @@ void frames_stale(IceHandle *h) {
     h->last_fused = false;
 }
 
+SurrArg surr_arg(const IceHandle *h) {
+    SurrArg a{};
+    a.M = MlpDev{h->mlp_pk, h->mlp_p};
+    a.N = h->mlp_norm;
+    a.tol = h->mlp_tol;
+    a.lj0 = (float)h->mlp_samp.logJ_min;
+    a.lj1 = (float)h->mlp_samp.logJ_max;
+    a.d0 = (float)h->mlp_samp.D_min;
+    a.d1 = (float)h->mlp_samp.D_max;
+    a.jp0 = (float)h->mlp_samp.jp_min;
+    a.jp1 = (float)h->mlp_samp.jp_max;
+    return a;
+}
+
 // Pipeline fill: P2G of the current point state into fr[fr_cur].
 int pipeline_fill(IceHandle *h) {
     const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
@@ -567,9 +595,9 @@ int pipeline_fill(IceHandle *h) {
     const StepArg S0{0.0, 0, nullptr, 0};
     float *F = h->fr[h->fr_cur];
     if (h->ntiles) {
-        hipLaunchKernelGGL(tk->fill, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, S0, h->d_desc,
-                           h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt, h->offtile,
-                           (const float *)F, F);
+        hipLaunchKernelGGL(tk->fill, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, S0, SurrArg{},
+                           h->d_desc, h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt,
+                           h->offtile, (const float *)F, F);
         LAUNCHCHK(h, "k_tile_pipeline (fill)");
     }
     hipLaunchKernelGGL(tk->offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S0, h->d_desc,
@@ -598,8 +626,10 @@ int phase_pipelined(IceHandle *h, const StepArg &S) {
     const float *Fc = h->fr[h->fr_cur];
     float *Fn = h->fr[h->fr_cur ^ 1];
     if (h->ntiles) {
-        hipLaunchKernelGGL(tk->fused, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, S, h->d_desc,
-                           h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt, h->offtile, Fc, Fn);
+        const bool surr = h->mlp_deployed && h->mlp_have_params && h->mlp_have_norm;
+        hipLaunchKernelGGL(surr ? tk->fused_surr : tk->fused, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp,
+                           S, surr ? surr_arg(h) : SurrArg{}, h->d_desc, h->d_tiles, h->ntiles, h->lines(),
+                           (const uint32_t *)h->cs, G, h->cnt, h->offtile, Fc, Fn);
         LAUNCHCHK(h, "k_tile_pipeline");
     }
     rec(h, 4);
@@ -993,7 +1023,9 @@ int ice_destroy(IceHandle *h) {
     for (void *p : {(void *)h->pts, (void *)h->pts_alt, (void *)h->grid, (void *)h->status, (void *)h->cnt,
                     (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
-                    (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1]})
+                    (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
+                    (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk,
+                    (void *)h->mlp_fail, (void *)h->mlp_x, (void *)h->mlp_t})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1429,6 +1461,316 @@ int ice_get_domain_info(IceHandle *h, int d, IceInfo *info) {
     int rc = read_counters(h);
     if (rc) return rc;
     fill_info(h, d, info);
+    return ICE_OK;
+}
+
+// ---- learned plastic projection ---------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+int mlp_alloc(IceHandle *h) {
+    if (h->mlp_p) return ICE_OK;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipMalloc(&h->mlp_p, sizeof(float) * kMlpParams));
+    HIPCHK(h, hipMalloc(&h->mlp_g, sizeof(float) * kMlpParams));
+    HIPCHK(h, hipMalloc(&h->mlp_m, sizeof(float) * kMlpParams));
+    HIPCHK(h, hipMalloc(&h->mlp_v, sizeof(float) * kMlpParams));
+    HIPCHK(h, hipMalloc(&h->mlp_pk, sizeof(bf16x4) * kPkOps * 64));
+    HIPCHK(h, hipMalloc(&h->mlp_fail, sizeof(uint32_t)));
+    int rc;
+    if ((rc = launch_fill(h, h->mlp_p, kMlpParams))) return rc;
+    if ((rc = launch_fill(h, h->mlp_m, kMlpParams))) return rc;
+    if ((rc = launch_fill(h, h->mlp_v, kMlpParams))) return rc;
+    ice_mlp_default_sampling(&h->mlp_samp);
+    return ICE_OK;
+}
+
+MlpPhys mlp_phys(const IceHandle *h) {
+    const IceParams &p = h->prm;
+    MlpPhys m;
+    m.kappa = p.kappa;
+    m.mu = p.mu;
+    m.ridge = p.RidgeFormationCoeff * 9.8 * p.IceDensity;
+    m.pmax = p.IceCompressiveStrength;
+    m.pmin = -p.IceTensileStrength;
+    m.ellipse_c = 2 * p.IceShearStrength / (m.pmax - m.pmin);
+    m.tan_phi = std::tan(p.DP_phi * 3.14159265358979323846 / 180);
+    m.thickness = 1.0;
+    return m;
+}
+
+int mlp_pack(IceHandle *h) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(kPkOps), dim3(64), 0, h->stream, (const float *)h->mlp_p, h->mlp_pk);
+    LAUNCHCHK(h, "k_mlp_pack");
+    return ICE_OK;
+}
+
+int mlp_ready(IceHandle *h) {
+    if (!h->mlp_have_params || !h->mlp_have_norm) {
+        h->err = "surrogate parameters and standardisation not set";
+        return ICE_E_STATE;
+    }
+    return ICE_OK;
+}
+
+// Device copy of n rows of `width` floats (freed by the caller).
+int mlp_upload(IceHandle *h, const float *src, size_t count, float **dst) {
+    *dst = nullptr;
+    HIPCHK(h, hipMalloc(dst, sizeof(float) * std::max<size_t>(count, 1)));
+    if (count) HIPCHK(h, hipMemcpyAsync(*dst, src, sizeof(float) * count, hipMemcpyHostToDevice, h->stream));
+    return ICE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ice_mlp_default_sampling(IceMlpSampling *s) {
+    if (!s) return ICE_E_ARG;
+    s->logJ_min = -0.02;
+    s->logJ_max = 0.0002;
+    s->D_min = 1e-4;
+    s->D_max = 2e-2;
+    s->jp_min = 0.5;
+    s->jp_max = 2.0;
+    return ICE_OK;
+}
+
+int ice_mlp_generate(IceHandle *h, uint32_t n, uint64_t seed, const IceMlpSampling *smp, float *x, float *t,
+                     uint32_t *n_failed) {
+    if (!h || n == 0) return ICE_E_ARG;
+    int rc = mlp_alloc(h);
+    if (rc) return rc;
+    if (smp) h->mlp_samp = *smp;
+    const IceMlpSampling &S = h->mlp_samp;
+    if (!(S.D_min > 0 && S.D_max > S.D_min && S.jp_min > 0 && S.jp_max > S.jp_min && S.logJ_max > S.logJ_min)) {
+        h->err = "bad sampling ranges";
+        return ICE_E_ARG;
+    }
+    if (h->mlp_x) (void)hipFree(h->mlp_x);
+    if (h->mlp_t) (void)hipFree(h->mlp_t);
+    h->mlp_x = h->mlp_t = nullptr;
+    h->mlp_n = 0;
+    HIPCHK(h, hipMalloc(&h->mlp_x, sizeof(float) * (size_t)n * kMlpIn));
+    HIPCHK(h, hipMalloc(&h->mlp_t, sizeof(float) * (size_t)n * kMlpOut));
+    if ((rc = launch_fill_u32(h, h->mlp_fail, 1, 0u))) return rc;
+    const MlpSampling ms{S.logJ_min, S.logJ_max, S.D_min, S.D_max, S.jp_min, S.jp_max};
+    hipLaunchKernelGGL(k_mlp_pairs, dim3(std::min<unsigned>(blocks_for(n, 256), 16384)), dim3(256), 0, h->stream,
+                       mlp_phys(h), ms, seed, n, h->mlp_x, h->mlp_t, h->mlp_fail);
+    LAUNCHCHK(h, "k_mlp_pairs");
+    h->mlp_n = n;
+    // standardisation from (up to) the first 2^18 pairs
+    const uint32_t ns = std::min<uint32_t>(n, 1u << 18);
+    std::vector<float> hx((size_t)ns * kMlpIn), ht((size_t)ns * kMlpOut);
+    HIPCHK(h, hipMemcpyAsync(hx.data(), h->mlp_x, hx.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ht.data(), h->mlp_t, ht.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    uint32_t failed = 0;
+    HIPCHK(h, hipMemcpyAsync(&failed, h->mlp_fail, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    auto stats = [&](const std::vector<float> &v, int w, int k, float &m, float &sd) {
+        double a = 0, b = 0;
+        for (uint32_t i = 0; i < ns; i++) a += v[(size_t)i * w + k];
+        a /= ns;
+        for (uint32_t i = 0; i < ns; i++) {
+            const double d = v[(size_t)i * w + k] - a;
+            b += d * d;
+        }
+        m = (float)a;
+        sd = (float)std::max(std::sqrt(b / ns), 1e-12);
+    };
+    for (int k = 0; k < kMlpIn; k++) stats(hx, kMlpIn, k, h->mlp_norm.xm[k], h->mlp_norm.xs[k]);
+    for (int k = 0; k < kMlpOut; k++) stats(ht, kMlpOut, k, h->mlp_norm.ym[k], h->mlp_norm.ys[k]);
+    h->mlp_have_norm = true;
+    if (x) HIPCHK(h, hipMemcpy(x, h->mlp_x, sizeof(float) * (size_t)n * kMlpIn, hipMemcpyDeviceToHost));
+    if (t) HIPCHK(h, hipMemcpy(t, h->mlp_t, sizeof(float) * (size_t)n * kMlpOut, hipMemcpyDeviceToHost));
+    if (n_failed) *n_failed = failed;
+    drop_graph(h);
+    return ICE_OK;
+}
+
+int ice_mlp_set_norm(IceHandle *h, const IceMlpNorm *norm) {
+    if (!h || !norm) return ICE_E_ARG;
+    static_assert(sizeof(IceMlpNorm) == sizeof(MlpNorm), "norm layout");
+    std::memcpy(&h->mlp_norm, norm, sizeof(MlpNorm));
+    h->mlp_have_norm = true;
+    drop_graph(h);
+    return ICE_OK;
+}
+
+int ice_mlp_get_norm(IceHandle *h, IceMlpNorm *norm) {
+    if (!h || !norm) return ICE_E_ARG;
+    std::memcpy(norm, &h->mlp_norm, sizeof(MlpNorm));
+    return ICE_OK;
+}
+
+int ice_mlp_set_params(IceHandle *h, const float *params) {
+    if (!h || !params) return ICE_E_ARG;
+    int rc = mlp_alloc(h);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->mlp_p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, h->stream));
+    if ((rc = launch_fill(h, h->mlp_m, kMlpParams)) || (rc = launch_fill(h, h->mlp_v, kMlpParams))) return rc;
+    if ((rc = mlp_pack(h))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->mlp_adam_t = 0;
+    h->mlp_have_params = true;
+    return ICE_OK;
+}
+
+int ice_mlp_get_params(IceHandle *h, float *params) {
+    if (!h || !params) return ICE_E_ARG;
+    if (!h->mlp_p) { h->err = "no surrogate parameters"; return ICE_E_STATE; }
+    HIPCHK(h, hipMemcpyAsync(params, h->mlp_p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ICE_OK;
+}
+
+int ice_mlp_init_params(IceHandle *h, uint64_t seed) {
+    if (!h) return ICE_E_ARG;
+    std::vector<float> p(kMlpParams, 0.f);
+    uint64_t st = seed * 0x9e3779b97f4a7c15ull + 1;
+    auto uni = [&]() {   // xorshift64*, uniform in [-1, 1)
+        st ^= st >> 12;
+        st ^= st << 25;
+        st ^= st >> 27;
+        return (double)((st * 0x2545f4914f6cdd1dull) >> 11) / 4503599627370496.0 - 1.0;
+    };
+    auto fill = [&](int off, int rows, int cols) {
+        const double a = std::sqrt(6.0 / (rows + cols));   // Glorot-uniform
+        for (int i = 0; i < rows * cols; i++) p[off + i] = (float)(a * uni());
+    };
+    fill(kOffW1, kMlpHid, kMlpIn);
+    fill(kOffW2, kMlpHid, kMlpHid);
+    fill(kOffW3, kMlpOut, kMlpHid);
+    return ice_mlp_set_params(h, p.data());
+}
+
+int ice_mlp_train(IceHandle *h, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *loss) {
+    if (!h || batch == 0 || !(lr > 0)) return ICE_E_ARG;
+    int rc = mlp_ready(h);
+    if (rc) return rc;
+    if (!h->mlp_n) { h->err = "no training pairs (ice_mlp_generate)"; return ICE_E_STATE; }
+    HIPCHK(h, hipSetDevice(h->device));
+    if (steps > h->mlp_loss_cap) {
+        if (h->mlp_loss) (void)hipFree(h->mlp_loss);
+        h->mlp_loss = nullptr;
+        HIPCHK(h, hipMalloc(&h->mlp_loss, sizeof(float) * steps));
+        h->mlp_loss_cap = steps;
+    }
+    if (steps && (rc = launch_fill(h, h->mlp_loss, steps))) return rc;
+    const MlpDev M{h->mlp_pk, h->mlp_p};
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (uint32_t k = 0; k < steps; k++) {
+        if ((rc = launch_fill(h, h->mlp_g, kMlpParams))) return rc;
+        hipLaunchKernelGGL(k_mlp_train, dim3(blocks_for(batch, 64)), dim3(64), 0, h->stream, M, h->mlp_norm,
+                           (const float *)h->mlp_x, (const float *)h->mlp_t, h->mlp_n, batch, seed, k, 0, h->mlp_g,
+                           h->mlp_loss + k);
+        LAUNCHCHK(h, "k_mlp_train");
+        h->mlp_adam_t++;
+        const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)h->mlp_adam_t)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)h->mlp_adam_t)));
+        hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, h->stream, h->mlp_p,
+                           (const float *)h->mlp_g, h->mlp_m, h->mlp_v, lr, b1, b2, eps, c1, c2);
+        LAUNCHCHK(h, "k_mlp_adam");
+        if ((rc = mlp_pack(h))) return rc;
+    }
+    if (loss && steps) HIPCHK(h, hipMemcpyAsync(loss, h->mlp_loss, sizeof(float) * steps, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graph(h);
+    return ICE_OK;
+}
+
+int ice_mlp_forward(IceHandle *h, const float *x, uint32_t n, float *y) {
+    if (!h || (!x && n) || (!y && n)) return ICE_E_ARG;
+    int rc = mlp_ready(h);
+    if (rc || n == 0) return rc;
+    float *dx = nullptr, *dy = nullptr;
+    if ((rc = mlp_upload(h, x, (size_t)n * kMlpIn, &dx))) return rc;
+    hipError_t e = hipMalloc(&dy, sizeof(float) * (size_t)n * kMlpOut);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mlp_forward, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p},
+                           h->mlp_norm, (const float *)dx, n, dy);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, dy, sizeof(float) * (size_t)n * kMlpOut, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) { h->err = std::string("ice_mlp_forward: ") + hipGetErrorString(e); return ICE_E_HIP; }
+    return ICE_OK;
+}
+
+int ice_mlp_grad(IceHandle *h, const float *x, const float *t, uint32_t n, float *grad, float *loss) {
+    if (!h || !x || !t || !grad || n == 0) return ICE_E_ARG;
+    int rc = mlp_ready(h);
+    if (rc) return rc;
+    float *dx = nullptr, *dt = nullptr, *dl = nullptr;
+    if ((rc = mlp_upload(h, x, (size_t)n * kMlpIn, &dx))) return rc;
+    if ((rc = mlp_upload(h, t, (size_t)n * kMlpOut, &dt))) { (void)hipFree(dx); return rc; }
+    hipError_t e = hipMalloc(&dl, sizeof(float));
+    if (e == hipSuccess && (rc = launch_fill(h, h->mlp_g, kMlpParams)) == ICE_OK && (rc = launch_fill(h, dl, 1)) == ICE_OK) {
+        hipLaunchKernelGGL(k_mlp_train, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p},
+                           h->mlp_norm, (const float *)dx, (const float *)dt, n, n, (uint64_t)0, 0u, 1, h->mlp_g, dl);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(grad, h->mlp_g, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dt);
+    if (dl) (void)hipFree(dl);
+    if (rc) return rc;
+    if (e != hipSuccess) { h->err = std::string("ice_mlp_grad: ") + hipGetErrorString(e); return ICE_E_HIP; }
+    return ICE_OK;
+}
+
+int ice_mlp_eval(IceHandle *h, const float *x, uint32_t n, float tol, float *out) {
+    if (!h || (!x && n) || (!out && n)) return ICE_E_ARG;
+    int rc = mlp_ready(h);
+    if (rc || n == 0) return rc;
+    float *dx = nullptr, *dout = nullptr;
+    if ((rc = mlp_upload(h, x, (size_t)n * kMlpIn, &dx))) return rc;
+    hipError_t e = hipMalloc(&dout, sizeof(float) * (size_t)n * 6);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mlp_eval, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p},
+                           h->mlp_norm, h->dp, mlp_phys(h), (const float *)dx, n, tol, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(float) * (size_t)n * 6, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dx);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) { h->err = std::string("ice_mlp_eval: ") + hipGetErrorString(e); return ICE_E_HIP; }
+    return ICE_OK;
+}
+
+int ice_mlp_deploy(IceHandle *h, int enable, float tol) {
+    if (!h || !(tol >= 0)) return ICE_E_ARG;
+    if (enable) {
+        int rc = mlp_ready(h);
+        if (rc) return rc;
+    }
+    h->mlp_deployed = enable != 0;
+    h->mlp_tol = tol;
+    for (int d = 0; d < h->ndom(); d++) {
+        const uint32_t zero2[2] = {0u, 0u};
+        HIPCHK(h, hipMemcpyAsync(&h->dcnt(d)->pad[0], zero2, sizeof(zero2), hipMemcpyHostToDevice, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graph(h);
+    return ICE_OK;
+}
+
+int ice_mlp_counts(IceHandle *h, uint64_t counts[2]) {
+    if (!h || !counts) return ICE_E_ARG;
+    int rc = read_counters(h);
+    if (rc) return rc;
+    counts[0] = counts[1] = 0;
+    for (int d = 0; d < h->ndom(); d++) {
+        counts[0] += h->host_cnt[d].pad[0];
+        counts[1] += h->host_cnt[d].pad[1];
+    }
     return ICE_OK;
 }
 

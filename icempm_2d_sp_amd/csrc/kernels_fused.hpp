@@ -24,6 +24,7 @@
 // with global atomics into the frame of the tile that now holds them.
 #pragma once
 #include "tile_common.hpp"
+#include "mlp.hpp"
 
 namespace ice {
 
@@ -206,8 +207,16 @@ __device__ __forceinline__ void stage_terms(float (*s_st)[kStage], uint32_t jj, 
 // FUSED = false: P2G of the current point state into frames Fn (fills the
 // pipeline).  FUSED = true: update + G2P of substep S from frames Fc, then P2G
 // of the new state into Fn.
-template <int TW, int TH, bool FUSED>
-__global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, StepArg S,
+// The surrogate's deployment arguments (used when SURR).
+struct SurrArg {
+    MlpDev M;
+    MlpNorm N;
+    float tol;
+    float lj0, lj1, d0, d1, jp0, jp1;   // feature ranges it was trained on
+};
+
+template <int TW, int TH, bool FUSED, bool SURR = false>
+__global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, StepArg S, SurrArg SA,
                                                                 const DomainDesc *__restrict__ D,
                                                                 const TileRef *__restrict__ T, int ntiles, Lines L0,
                                                                 const uint32_t *__restrict__ cs0, GridLines G0,
@@ -331,25 +340,28 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                     bool stage = false;
                     P2GPoint q;
                     const uint32_t i = slot[p];
-                    if (i != ~0u && !(raw[p].u & (kDisabled | kEscaped))) {
-                        G2PState s;
-                        s.u = raw[p].u;
-                        s.cx = (int)(raw[p].cell & 0xffff);
-                        s.cy = (int)(raw[p].cell >> 16);
-                        s.x = raw[p].x;
-                        s.y = raw[p].y;
-                        s.Fe = M2{raw[p].f00, raw[p].f01, raw[p].f10, raw[p].f11};
-                        s.Jp = raw[p].jp;
-                        const float th = raw[p].th;
+                    bool act = i != ~0u && !(raw[p].u & (kDisabled | kEscaped));
+                    G2PState s;
+                    s.u = raw[p].u;
+                    s.cx = (int)(raw[p].cell & 0xffff);
+                    s.cy = (int)(raw[p].cell >> 16);
+                    s.x = raw[p].x;
+                    s.y = raw[p].y;
+                    s.Fe = M2{raw[p].f00, raw[p].f01, raw[p].f10, raw[p].f11};
+                    s.Jp = raw[p].jp;
+                    const float th = raw[p].th;
+                    float pvx = 0.f, pvy = 0.f;
+                    M2 C{0.f, 0.f, 0.f, 0.f};
+                    FInv vf;
+                    if (act) {
                         const int rx = s.cx - fx0, ry = s.cy - fy0;
                         if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2) {
                             err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
+                            act = false;
                         } else {
                             float wx[3], wy[3];
                             weights(s.x, wx);
                             weights(s.y, wy);
-                            float pvx = 0.f, pvy = 0.f;
-                            M2 C{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                             for (int a = -1; a <= 1; a++)
 #pragma unroll
@@ -365,28 +377,70 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                                     C.a10 += f * (nv.y * dx);
                                     C.a11 += f * (nv.y * dy);
                                 }
-                            FInv vf;
-                            g2p_finish(P, s, pvx, pvy, C, th, raw[p].dpt, err, vf);
-                            if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
-                            bool escaped = false;
-                            if (s.cell_updated && !(s.u & kDisabled)) {
-                                if (s.u & kMoved) {
-                                    s.u |= kEscaped | kFresh;
-                                    const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
-                                    offtile[e] = i;
-                                    escaped = true;
-                                } else {
-                                    s.u |= kMoved;
+                            g2p_predict(P, s, pvx, pvy, C, err, vf);
+                        }
+                    }
+                    bool exact_rm = act && (s.u & kCrushed);
+                    if constexpr (SURR) {
+                        // the plastic branch through the learned projection, one
+                        // wave of points per MFMA batch (all lanes take part)
+                        // the plastic branch with the ridge cap active (what it was trained on)
+                        bool want = exact_rm && th == 1.f && s.p_tr >= raw[p].dpt &&
+                                    s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, raw[p].dpt, P);
+                        float xr[kMlpIn] = {0.f, 0.f, 1.f, 0.f};
+                        if (want) {
+                            mlp_features(vf, s.Jp, -raw[p].dpt * P.tan_phi, xr);
+                            want = xr[0] >= SA.lj0 && xr[0] <= SA.lj1 && xr[1] >= SA.d0 && xr[1] <= SA.d1 &&
+                                   xr[2] >= SA.jp0 && xr[2] <= SA.jp1;
+                        }
+                        if (__any(want)) {
+                            float xs[kMlpIn];
+#pragma unroll
+                            for (int k = 0; k < kMlpIn; k++) xs[k] = want ? (xr[k] - SA.N.xm[k]) / SA.N.xs[k] : 0.f;
+                            MlpFwd F;
+                            mlp_forward(SA.M, xs, F);
+                            float y[kMlpOut];
+                            mlp_lane_outputs(F, y);
+                            bool accepted = false;
+                            if (want) {
+                                float s0, s1, jpf, pq, qq, viol;
+                                accepted = mlp_decode(SA.N, y, s.Jp, raw[p].dpt, th, P, SA.tol, s0, s1, jpf, pq, qq, viol);
+                                if (accepted) {
+                                    s.Fe = rebuild(vf, s0, s1);
+                                    s.Jp *= jpf;
+                                    vf = invariants(s.Fe);
+                                    if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
+                                    exact_rm = false;
                                 }
                             }
-                            // escaped and disabled points leave the pipeline: store all
-                            g2p_store_pipelined(L, i, s, raw[p].u, raw[p].jp, pvx, pvy, C, rec,
-                                                S.complete || escaped || (s.u & kDisabled));
-                            if (!(s.u & kDisabled) && !escaped) {
-                                p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);
-                                if (s.u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
-                                else stage = true;
+                            const uint64_t ba = __ballot(accepted), bw = __ballot(want);
+                            if (lane == 0) {
+                                atomicAdd(&cnt->pad[0], (uint32_t)__popcll(ba));
+                                atomicAdd(&cnt->pad[1], (uint32_t)__popcll(bw & ~ba));
                             }
+                        }
+                    }
+                    if (act) {
+                        if (exact_rm) g2p_return_map(P, s, th, raw[p].dpt, err, vf);
+                        if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
+                        bool escaped = false;
+                        if (s.cell_updated && !(s.u & kDisabled)) {
+                            if (s.u & kMoved) {
+                                s.u |= kEscaped | kFresh;
+                                const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
+                                offtile[e] = i;
+                                escaped = true;
+                            } else {
+                                s.u |= kMoved;
+                            }
+                        }
+                        // escaped and disabled points leave the pipeline: store all
+                        g2p_store_pipelined(L, i, s, raw[p].u, raw[p].jp, pvx, pvy, C, rec,
+                                            S.complete || escaped || (s.u & kDisabled));
+                        if (!(s.u & kDisabled) && !escaped) {
+                            p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);
+                            if (s.u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
+                            else stage = true;
                         }
                     }
                     stage_terms(s_st, jj, stage, q);

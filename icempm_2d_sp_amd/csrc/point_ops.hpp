@@ -109,9 +109,11 @@ struct G2PState {
     bool newly_disabled;
 };
 
-// vf receives the invariants of the final Fe (for the next P2G's stress).
-__device__ __forceinline__ void g2p_finish(const DevParams &P, G2PState &s, float pvx, float pvy, const M2 &C,
-                                           float thickness, float dpt, uint32_t &err, FInv &vf) {
+// G2P after the stencil gather up to the return map (kernels.cuh:239-283):
+// advect, cell crossing and disable band, F update, PQ, the crushed latch.  vf
+// receives the invariants of the trial Fe.
+__device__ __forceinline__ void g2p_predict(const DevParams &P, G2PState &s, float pvx, float pvy, const M2 &C,
+                                            uint32_t &err, FInv &vf) {
     s.x += pvx * P.dt_h_inv;
     s.y += pvy * P.dt_h_inv;
     if (__builtin_isnan(pvx) || __builtin_isnan(pvy)) err |= ICE_ERR_POINT_VEL_NAN;
@@ -139,10 +141,22 @@ __device__ __forceinline__ void g2p_finish(const DevParams &P, G2PState &s, floa
     vf = invariants(s.Fe);
     compute_pq(vf, P, s.p_tr, s.q_tr);
     if (!(s.u & kCrushed) && outside_failure_surface(s.p_tr, s.q_tr, P)) s.u |= kCrushed;
-    if (s.u & kCrushed) {
-        if (return_map(vf, s.p_tr, s.q_tr, thickness, dpt, P, s.Fe, s.Jp) != 2) vf = invariants(s.Fe);
-        if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
-    }
+}
+
+// The return map of a crushed point (kernels.cuh:284-285); vf is updated to
+// the invariants of the final Fe.
+__device__ __forceinline__ void g2p_return_map(const DevParams &P, G2PState &s, float thickness, float dpt,
+                                               uint32_t &err, FInv &vf) {
+    if (return_map(vf, s.p_tr, s.q_tr, thickness, dpt, P, s.Fe, s.Jp) != 2) vf = invariants(s.Fe);
+    if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
+}
+
+// Everything G2P does after the stencil gather (kernels.cuh:239-314); vf
+// receives the invariants of the final Fe (for the next P2G's stress).
+__device__ __forceinline__ void g2p_finish(const DevParams &P, G2PState &s, float pvx, float pvy, const M2 &C,
+                                           float thickness, float dpt, uint32_t &err, FInv &vf) {
+    g2p_predict(P, s, pvx, pvy, C, err, vf);
+    if (s.u & kCrushed) g2p_return_map(P, s, thickness, dpt, err, vf);
 }
 
 __device__ __forceinline__ float dp_threshold(const DevParams &P, const Lines &L, uint32_t i) {

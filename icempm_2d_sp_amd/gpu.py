@@ -293,7 +293,76 @@ class GPU_Implementation5:
             p.record_timings(enablePointTransfer)
 
 
-class Simulation:
+class SurrogateMixin:
+    """The learned plastic projection's entry points (ice_mlp_*) on a handle."""
+
+    def mlp_generate(self, n: int, seed: int = 1, sampling: "abi.IceMlpSampling | None" = None, copy: bool = False):
+        x = np.zeros((n, 4), np.float32) if copy else None
+        t = np.zeros((n, 3), np.float32) if copy else None
+        failed = C.c_uint32(0)
+        self._chk(self.lib.ice_mlp_generate(self.h, n, seed, C.byref(sampling) if sampling is not None else None,
+                                            abi.ptr(x), abi.ptr(t), C.byref(failed)), "ice_mlp_generate")
+        return x, t, failed.value
+
+    def mlp_norm(self) -> "abi.IceMlpNorm":
+        nrm = abi.IceMlpNorm()
+        self._chk(self.lib.ice_mlp_get_norm(self.h, C.byref(nrm)), "ice_mlp_get_norm")
+        return nrm
+
+    def mlp_set_norm(self, nrm: "abi.IceMlpNorm"):
+        self._chk(self.lib.ice_mlp_set_norm(self.h, C.byref(nrm)), "ice_mlp_set_norm")
+
+    def mlp_init(self, seed: int = 1):
+        self._chk(self.lib.ice_mlp_init_params(self.h, seed), "ice_mlp_init_params")
+
+    def mlp_params(self) -> np.ndarray:
+        p = np.zeros(abi.MLP_PARAMS, np.float32)
+        self._chk(self.lib.ice_mlp_get_params(self.h, abi.ptr(p)), "ice_mlp_get_params")
+        return p
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (abi.MLP_PARAMS,)
+        self._chk(self.lib.ice_mlp_set_params(self.h, abi.ptr(p)), "ice_mlp_set_params")
+
+    def mlp_train(self, steps: int, batch: int = 8192, lr: float = 1e-3, seed: int = 7) -> np.ndarray:
+        loss = np.zeros(max(steps, 1), np.float32)
+        self._chk(self.lib.ice_mlp_train(self.h, steps, batch, lr, seed, abi.ptr(loss)), "ice_mlp_train")
+        return loss[:steps]
+
+    def mlp_forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        y = np.zeros((x.shape[0], 3), np.float32)
+        self._chk(self.lib.ice_mlp_forward(self.h, abi.ptr(x), x.shape[0], abi.ptr(y)), "ice_mlp_forward")
+        return y
+
+    def mlp_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 3)
+        g = np.zeros(abi.MLP_PARAMS, np.float32)
+        loss = C.c_float(0)
+        self._chk(self.lib.ice_mlp_grad(self.h, abi.ptr(x), abi.ptr(t), x.shape[0], abi.ptr(g), C.byref(loss)),
+                  "ice_mlp_grad")
+        return float(loss.value), g
+
+    def mlp_eval(self, x, tol: float = 1e-4) -> np.ndarray:
+        """[n][6]: surrogate p', q'; exact p', q'; yield violation; accepted."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 4)
+        out = np.zeros((x.shape[0], 6), np.float32)
+        self._chk(self.lib.ice_mlp_eval(self.h, abi.ptr(x), x.shape[0], tol, abi.ptr(out)), "ice_mlp_eval")
+        return out
+
+    def mlp_deploy(self, enable: bool = True, tol: float = 1e-4):
+        self._chk(self.lib.ice_mlp_deploy(self.h, int(enable), tol), "ice_mlp_deploy")
+
+    def mlp_counts(self):
+        c = (C.c_uint64 * 2)()
+        self._chk(self.lib.ice_mlp_counts(self.h, c), "ice_mlp_counts")
+        return int(c[0]), int(c[1])
+
+
+
+class Simulation(SurrogateMixin):
     """A scene on one GPU through the C-ABI: upload, run, download."""
 
     def __init__(self, scene, device: int = 0, *, flags: int | None = None, tile=None, capacity: int | None = None,
@@ -393,7 +462,9 @@ class Simulation:
         return (tw.value, th.value), ms.value
 
 
-class BatchSimulation:
+
+
+class BatchSimulation(SurrogateMixin):
     """Several independent scenes in one batched handle (ice_create_batch):
     one launch per phase over all domains' tiles, HIP-graph replays."""
 

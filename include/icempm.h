@@ -241,6 +241,60 @@ int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy);
  * (write `bytes`). */
 int ice_stream_bench(int device, size_t bytes, int reps, int mode, double *ms_per_pass);
 
+/* ---- learned plastic projection (no reference counterpart; replaces, when
+ * deployed, the plastic branch of Wolper_Drucker_Prager, kernels.cuh:431-479).
+ * A 4 -> 64 -> 64 -> 3 MLP with SiLU, BF16 weights and activations on MFMA,
+ * FP32 accumulation.  Features of a trial state: log J, D = log(s0/s1),
+ * Jp_inv, log(c / 1e5) (s0 >= s1 the singular values of the elastic
+ * predictor, c the cohesion); targets: the projected log J', D' and
+ * log(Jp_inv' / Jp_inv).  Parameters: W1[64][4], b1[64], W2[64][64], b2[64],
+ * W3[3][64], b3[3] (ICE_MLP_PARAMS floats). */
+#define ICE_MLP_PARAMS 4675
+typedef struct IceMlpNorm {
+    float xm[4], xs[4];          /* standardised feature = (raw - xm) / xs */
+    float ym[3], ys[3];          /* raw target = ym + ys * output */
+} IceMlpNorm;
+typedef struct IceMlpSampling {  /* trial states for training pairs */
+    double logJ_min, logJ_max;   /* uniform */
+    double D_min, D_max;         /* log-uniform */
+    double jp_min, jp_max;       /* log-uniform; c ~ lognormal(log 1e5, 0.5), capped at 1e6 */
+} IceMlpSampling;
+/* Sampling ranges used when `s` is NULL. */
+int ice_mlp_default_sampling(IceMlpSampling *s);
+/* n training pairs on the device from the exact projection in FP64 with the
+ * handle's constants (thickness 1, ridge cap active), kept for ice_mlp_train; copies to x
+ * [n][4] and t [n][3] when not NULL.  Sets the standardisation from the pairs.
+ * *n_failed = pairs whose sampling found no plastic state. */
+int ice_mlp_generate(IceHandle *h, uint32_t n, uint64_t seed, const IceMlpSampling *s, float *x, float *t,
+                     uint32_t *n_failed);
+int ice_mlp_set_norm(IceHandle *h, const IceMlpNorm *norm);
+int ice_mlp_get_norm(IceHandle *h, IceMlpNorm *norm);
+/* FP32 master parameters (ICE_MLP_PARAMS). ice_mlp_init_params draws
+ * scaled-uniform initial weights from `seed`. */
+int ice_mlp_set_params(IceHandle *h, const float *params);
+int ice_mlp_get_params(IceHandle *h, float *params);
+int ice_mlp_init_params(IceHandle *h, uint64_t seed);
+/* `steps` Adam steps on batches of `batch` pairs drawn from the generated
+ * pairs (mean squared error of the standardised targets); loss per step into
+ * `loss` when not NULL. */
+int ice_mlp_train(IceHandle *h, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *loss);
+/* Forward pass over n raw feature rows x [n][4]: standardised outputs y [n][3]. */
+int ice_mlp_forward(IceHandle *h, const float *x, uint32_t n, float *y);
+/* Gradient of the mean squared error over the n rows (x raw, t raw) into
+ * grad (ICE_MLP_PARAMS), loss into *loss; parameters unchanged. */
+int ice_mlp_grad(IceHandle *h, const float *x, const float *t, uint32_t n, float *grad, float *loss);
+/* Per raw trial state x [n][4]: out [n][6] = surrogate (p', q'), exact FP64
+ * (p', q'), yield violation |q' - q_yield(p')| / q_yield(p') of the
+ * surrogate's state, 1 if the surrogate is accepted at tolerance `tol`. */
+int ice_mlp_eval(IceHandle *h, const float *x, uint32_t n, float tol, float *out);
+/* Use the surrogate for the plastic projection in pipelined substeps
+ * (enable != 0), falling back to the exact map when its yield violation
+ * exceeds `tol`, its features leave the training ranges or thickness != 1.
+ * counts[0] = projections the surrogate made, counts[1] = fallbacks, summed
+ * over domains since the last ice_mlp_deploy. */
+int ice_mlp_deploy(IceHandle *h, int enable, float tol);
+int ice_mlp_counts(IceHandle *h, uint64_t counts[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,8 @@ def load():
         "orc_g2p": (C.c_int, [P, C.c_int]),
         "orc_sort": (C.c_int, [P, C.POINTER(C.c_uint32), P]),
         "orc_params_derive": (None, [PP]),
+        "orc_mlp_forward": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
+        "orc_mlp_grad": (C.c_double, [P, P, P, P, C.c_int, C.c_int, P]),
     }
     for suf, T in (("f64", C.c_double), ("f32", C.c_float)):
         TP = C.POINTER(T)
@@ -248,3 +250,32 @@ def constitutive(params, F, Jp: float, util: int, thickness: float = 1.0, dp_thr
     case = getattr(lib, f"orc_constitutive_f{prec}")(C.byref(cp), dpt, thickness, Fa, C.byref(jp), C.byref(u),
                                                      C.byref(p_tr), C.byref(q_tr), C.byref(err))
     return case, np.array(list(Fa)).reshape(2, 2), jp.value, u.value, p_tr.value, q_tr.value, err.value
+
+
+# ---- surrogate network (mlp_oracle.hpp) ----
+
+def _norm_vec(norm) -> np.ndarray:
+    """IceMlpNorm-like object or dict -> [xm(4), xs(4), ym(3), ys(3)] float64."""
+    if isinstance(norm, dict):
+        return np.ascontiguousarray(np.concatenate([norm["xm"], norm["xs"], norm["ym"], norm["ys"]]), np.float64)
+    return np.ascontiguousarray(np.concatenate([list(norm.xm), list(norm.xs), list(norm.ym), list(norm.ys)]),
+                                np.float64)
+
+
+def mlp_forward(params, norm, x, bf16: bool = True) -> np.ndarray:
+    """Standardised outputs [n][3] of the surrogate network for raw features x [n][4]."""
+    p = np.ascontiguousarray(params, np.float64)
+    xx = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
+    y = np.zeros((xx.shape[0], 3))
+    load().orc_mlp_forward(_p(p), _p(_norm_vec(norm)), _p(xx), xx.shape[0], int(bf16), _p(y))
+    return y
+
+
+def mlp_grad(params, norm, x, t, bf16: bool = True):
+    """(loss, gradient [4675]) of the mean squared error of standardised targets."""
+    p = np.ascontiguousarray(params, np.float64)
+    xx = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
+    tt = np.ascontiguousarray(t, np.float64).reshape(-1, 3)
+    g = np.zeros(p.shape[0])
+    loss = load().orc_mlp_grad(_p(p), _p(_norm_vec(norm)), _p(xx), _p(tt), xx.shape[0], int(bf16), _p(g))
+    return float(loss), g

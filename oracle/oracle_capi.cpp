@@ -1,6 +1,7 @@
 // oracle_capi.cpp — extern "C" face of the oracle for the Python tests and the
 // CPU baseline.  TEST INFRASTRUCTURE ONLY: never linked into the product.
 #include "mpm_oracle.hpp"
+#include "mlp_oracle.hpp"
 #include <chrono>
 
 using namespace orc;
@@ -271,5 +272,28 @@ void orc_params_derive(IceParams *p) {
 
 FUNCS(double, f64)
 FUNCS(float, f32)
+
+// ---- surrogate network (mlp_oracle.hpp) ----
+// x raw [n][4]; norm = {xm[4], xs[4], ym[3], ys[3]}; y standardised outputs [n][3].
+int orc_mlp_forward(const double *params, const double *norm, const double *x, int n, int bf16, double *y) {
+    orc_mlp::Norm N;
+    std::memcpy(&N, norm, sizeof(N));
+    orc_mlp::Trace T;
+    for (int s = 0; s < n; s++) {
+        double xs[orc_mlp::IN];
+        for (int k = 0; k < orc_mlp::IN; k++) xs[k] = (x[s * orc_mlp::IN + k] - N.xm[k]) / N.xs[k];
+        orc_mlp::forward(params, xs, bf16 != 0, T);
+        for (int k = 0; k < orc_mlp::OUT; k++) y[s * orc_mlp::OUT + k] = T.y[k];
+    }
+    return 0;
+}
+
+double orc_mlp_grad(const double *params, const double *norm, const double *x, const double *t, int n, int bf16,
+                    double *grad) {
+    orc_mlp::Norm N;
+    std::memcpy(&N, norm, sizeof(N));
+    std::memset(grad, 0, sizeof(double) * orc_mlp::NPARAM);
+    return orc_mlp::loss_grad(params, N, x, t, n, bf16 != 0, grad);
+}
 
 }  // extern "C"
