@@ -107,8 +107,10 @@ def main() -> int:
     ap.add_argument("--naive", action="store_true", help="reference launch structure (unsorted, atomics)")
     ap.add_argument("--unfused", action="store_true", help="separate P2G and update + G2P tile kernels")
     ap.add_argument("--scale", type=float, default=1.0, help="linear domain scale (1.0 = 4096x2048)")
-    ap.add_argument("--workload", choices=["icefield_4096x2048", "batch"], default="icefield_4096x2048",
-                    help="icefield: one 4096x2048 domain (the headline); batch64: 64 uneven domains")
+    ap.add_argument("--workload", choices=["icefield", "batch64", "surrogate"], default="icefield",
+                    help="icefield: one 4096x2048 domain (the headline); batch64: 64 uneven domains; "
+                         "surrogate: the 64 domains under high-strain-rate forcing, plain vs learned projection")
+    ap.add_argument("--surrogate-tol", type=float, default=1e-4, help="yield-violation tolerance of the fallback")
     ap.add_argument("--copy-only", type=float, default=0.0,
                     help="only time a plain copy of this many GB (HBM reference / counter calibration)")
     args = ap.parse_args()
@@ -135,6 +137,8 @@ def main() -> int:
 
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
 
     gx, gy = int(round(4096 * args.scale)), int(round(2048 * args.scale))
     t0 = time.time()
@@ -285,6 +289,120 @@ def bench_batch(args) -> int:
         "batched_over_sequential": wall_s / wall_b,
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_surrogate(args) -> int:
+    """batch64_compressed: 64 uneven domains in uniaxial compression at 1/s
+    under 1 Pa wind (most points project plastically every substep).  Times
+    (a) plain batched substeps (exact return map), (b) batched substeps with
+    the learned plastic projection and the exact fallback at
+    --surrogate-tol, and (c) the plain single-domain runs one after another.
+    The network is trained on the GPU first (not timed); its stress error
+    against the FP64 exact projection is measured on held-out trial states.
+    value = (b)."""
+    import numpy as np
+    from icempm_2d_sp_amd import abi, scenes
+    from icempm_2d_sp_amd.gpu import BatchSimulation, Simulation
+    doms = scenes.batch64_compressed()
+    steps = max(16, (args.steps // 16) * 16)
+    tol = args.surrogate_tol
+    print(f"[bench] {len(doms)} domains, {sum(d.n for d in doms)} points; {steps} substeps", file=sys.stderr,
+          flush=True)
+
+    def timed(sim):
+        sim.run(0, 16, graph=True)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.run(16, steps, graph=True)
+        sim.sync()
+        return time.perf_counter() - t0
+
+    plain = BatchSimulation(doms)
+    n_active = plain.n_points()
+    wall_plain = timed(plain)
+    print(f"[bench] plain batched {wall_plain:.3f}s", file=sys.stderr, flush=True)
+
+    surr = BatchSimulation(doms)
+    t0 = time.perf_counter()
+    surr.mlp_generate(1 << 20, seed=3)
+    surr.mlp_init(seed=21)
+    loss = np.concatenate([surr.mlp_train(2000, batch=8192, lr=2e-3, seed=5),
+                           surr.mlp_train(2000, batch=8192, lr=4e-4, seed=6)])
+    t_train = time.perf_counter() - t0
+    trained, trained_norm = surr.mlp_params(), surr.mlp_norm()
+    print(f"[bench] trained in {t_train:.2f}s, loss {loss[0]:.3g} -> {loss[-100:].mean():.3g}", file=sys.stderr,
+          flush=True)
+    # a looser tolerance first (reported beside), then the requested one (value)
+    loose = 1e-2
+    surr.mlp_deploy(True, loose)
+    wall_loose = timed(surr)
+    acc_loose, fb_loose = surr.mlp_counts()
+    surr.close()
+    surr = BatchSimulation(doms)
+    surr.mlp_set_norm(trained_norm)
+    surr.mlp_set_params(trained)
+    surr.mlp_deploy(True, tol)
+    wall_surr = timed(surr)
+    n_acc, n_fb = surr.mlp_counts()
+    print(f"[bench] surrogate batched {wall_surr:.3f}s, projections {n_acc}, fallbacks {n_fb}; at tol {loose:g}: "
+          f"{wall_loose:.3f}s, projections {acc_loose}, fallbacks {fb_loose}", file=sys.stderr, flush=True)
+    # accuracy on held-out trial states (another seed, another handle's pairs)
+    aux = Simulation(doms[0])
+    x_test, _, _ = aux.mlp_generate(1 << 16, seed=99, copy=True)
+    aux.close()
+    out = surr.mlp_eval(x_test, tol)
+    ps, qs, pe, qe, viol, acc = (out[:, k].astype(np.float64) for k in range(6))
+    err = np.hypot(ps - pe, qs - qe) / np.hypot(pe, qe)
+    accepted = acc > 0.5
+    err_deployed = np.where(accepted, err, 0.0)   # fallback points take the exact map
+    # final state of the first domain: surrogate run vs plain run
+    pa, _, _, _ = plain.download(0)
+    pb, _, _, _ = surr.download(0)
+    ka = np.argsort(pa[abi.IDX_POINT].view(np.uint32), kind="stable")
+    kb = np.argsort(pb[abi.IDX_POINT].view(np.uint32), kind="stable")
+    fe_a = pa[abi.IDX_FE00:abi.IDX_FE00 + 4][:, ka].astype(np.float64)
+    fe_b = pb[abi.IDX_FE00:abi.IDX_FE00 + 4][:, kb].astype(np.float64)
+    dfe = float(np.abs(fe_a - fe_b).max())
+    plain.close()
+    surr.close()
+    # plain single-domain runs, one after another (the reference's launch shape)
+    wall_seq = 0.0
+    for sc in doms:
+        s1 = Simulation(sc)
+        wall_seq += timed(s1)
+        s1.close()
+    print(f"[bench] plain sequential {wall_seq:.3f}s", file=sys.stderr, flush=True)
+    plastic = (n_acc + n_fb) / (n_active * steps)
+    outj = {
+        "metric": "particle-updates/s per substep, 64 batched uneven domains, surrogate + batched vs plain",
+        "value": n_active * steps / wall_surr,
+        "unit": "particle-updates/s",
+        "n_gpus": 1, "steps": steps, "warmup": 16, "ms_per_step": wall_surr * 1000 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 state, bf16 MLP",
+        "data": "synthetic",
+        "config": {"workload": "batch64_compressed: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, "
+                               "uniaxial compression 1/s toward x_max, 1.0 Pa wind along +x, 4 points/cell",
+                   "points": n_active, "surrogate_tol": tol},
+        "plain_batched": {"value": n_active * steps / wall_plain, "ms_per_step": wall_plain * 1000 / steps},
+        "plain_sequential": {"value": n_active * steps / wall_seq, "ms_per_step": wall_seq * 1000 / steps},
+        "surrogate_batched_over_plain_sequential": wall_seq / wall_surr,
+        "surrogate_batched_over_plain_batched": wall_plain / wall_surr,
+        "ridge_capped_plastic_fraction_per_substep": plastic,
+        "surrogate_accepted_fraction": n_acc / max(n_acc + n_fb, 1),
+        "at_tol_1e-2": {"value": n_active * steps / wall_loose, "ms_per_step": wall_loose * 1000 / steps,
+                        "accepted_fraction": acc_loose / max(acc_loose + fb_loose, 1),
+                        "over_plain_sequential": wall_seq / wall_loose, "over_plain_batched": wall_plain / wall_loose},
+        "training": {"pairs": 1 << 20, "steps": int(loss.shape[0]), "seconds": t_train,
+                     "loss_first": float(loss[0]), "loss_last": float(loss[-100:].mean())},
+        "stress_error_vs_fp64": {
+            "network_median": float(np.median(err)), "network_p99": float(np.percentile(err, 99)),
+            "network_max": float(err.max()),
+            "accepted_fraction_heldout": float(accepted.mean()),
+            "deployed_max": float(err_deployed.max()), "deployed_mean": float(err_deployed.mean())},
+        "final_Fe_max_abs_diff_first_domain": dfe,
+    }
+    print(json.dumps(outj), flush=True)
     return 0
 
 

@@ -465,8 +465,12 @@ void rec(IceHandle *h, int k) {
     if (t) (void)hipEventRecord(t->e[k], h->stream);
 }
 
+// Workgroups per domain of the off-tile kernels (grid-stride over the list,
+// whose length only the device knows): up to one per 2048 slots of the
+// largest domain, so a long list (fast flows between re-sorts) is spread over
+// the whole GPU; blocks past the list's end exit after one load.
 unsigned offtile_blocks(const IceHandle *h) {
-    return std::min<unsigned>(64, blocks_for(h->max_capacity(), 4096));
+    return std::min<unsigned>(1024, blocks_for(h->max_capacity(), 2048));
 }
 
 DevParams dom_params(const IceHandle *h, int d) {

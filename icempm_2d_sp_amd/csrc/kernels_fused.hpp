@@ -216,7 +216,7 @@ struct SurrArg {
 };
 
 template <int TW, int TH, bool FUSED, bool SURR = false>
-__global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, StepArg S, SurrArg SA,
+__global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0, StepArg S, SurrArg SA,
                                                                 const DomainDesc *__restrict__ D,
                                                                 const TileRef *__restrict__ T, int ntiles, Lines L0,
                                                                 const uint32_t *__restrict__ cs0, GridLines G0,
@@ -329,27 +329,36 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                 }
             }
             if constexpr (FUSED) {
-                // every pass's loads first, then the computation
+                // every pass's loads first, then the computation (the surrogate
+                // variant, short of registers, loads each pass when it computes it)
                 G2PRaw raw[kPasses];
+                if constexpr (!SURR) {
 #pragma unroll
-                for (int p = 0; p < kPasses; p++)
-                    if (slot[p] != ~0u) load_g2p_raw(L, P, slot[p], raw[p]);
+                    for (int p = 0; p < kPasses; p++)
+                        if (slot[p] != ~0u) load_g2p_raw(L, P, slot[p], raw[p]);
+                }
 #pragma unroll
                 for (int p = 0; p < kPasses; p++) {
                     const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;   // staging slot
                     bool stage = false;
                     P2GPoint q;
                     const uint32_t i = slot[p];
-                    bool act = i != ~0u && !(raw[p].u & (kDisabled | kEscaped));
+                    G2PRaw rw;
+                    if constexpr (SURR) {
+                        if (i != ~0u) load_g2p_raw(L, P, i, rw);
+                    } else {
+                        rw = raw[p];
+                    }
+                    bool act = i != ~0u && !(rw.u & (kDisabled | kEscaped));
                     G2PState s;
-                    s.u = raw[p].u;
-                    s.cx = (int)(raw[p].cell & 0xffff);
-                    s.cy = (int)(raw[p].cell >> 16);
-                    s.x = raw[p].x;
-                    s.y = raw[p].y;
-                    s.Fe = M2{raw[p].f00, raw[p].f01, raw[p].f10, raw[p].f11};
-                    s.Jp = raw[p].jp;
-                    const float th = raw[p].th;
+                    s.u = rw.u;
+                    s.cx = (int)(rw.cell & 0xffff);
+                    s.cy = (int)(rw.cell >> 16);
+                    s.x = rw.x;
+                    s.y = rw.y;
+                    s.Fe = M2{rw.f00, rw.f01, rw.f10, rw.f11};
+                    s.Jp = rw.jp;
+                    const float th = rw.th;
                     float pvx = 0.f, pvy = 0.f;
                     M2 C{0.f, 0.f, 0.f, 0.f};
                     FInv vf;
@@ -385,11 +394,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                         // the plastic branch through the learned projection, one
                         // wave of points per MFMA batch (all lanes take part)
                         // the plastic branch with the ridge cap active (what it was trained on)
-                        bool want = exact_rm && th == 1.f && s.p_tr >= raw[p].dpt &&
-                                    s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, raw[p].dpt, P);
+                        bool want = exact_rm && th == 1.f && s.p_tr >= rw.dpt &&
+                                    s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, rw.dpt, P);
                         float xr[kMlpIn] = {0.f, 0.f, 1.f, 0.f};
                         if (want) {
-                            mlp_features(vf, s.Jp, -raw[p].dpt * P.tan_phi, xr);
+                            mlp_features(vf, s.Jp, -rw.dpt * P.tan_phi, xr);
                             want = xr[0] >= SA.lj0 && xr[0] <= SA.lj1 && xr[1] >= SA.d0 && xr[1] <= SA.d1 &&
                                    xr[2] >= SA.jp0 && xr[2] <= SA.jp1;
                         }
@@ -397,14 +406,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                             float xs[kMlpIn];
 #pragma unroll
                             for (int k = 0; k < kMlpIn; k++) xs[k] = want ? (xr[k] - SA.N.xm[k]) / SA.N.xs[k] : 0.f;
-                            MlpFwd F;
-                            mlp_forward(SA.M, xs, F);
                             float y[kMlpOut];
-                            mlp_lane_outputs(F, y);
+                            mlp_infer(SA.M, xs, y);
                             bool accepted = false;
                             if (want) {
                                 float s0, s1, jpf, pq, qq, viol;
-                                accepted = mlp_decode(SA.N, y, s.Jp, raw[p].dpt, th, P, SA.tol, s0, s1, jpf, pq, qq, viol);
+                                accepted = mlp_decode(SA.N, y, s.Jp, rw.dpt, th, P, SA.tol, s0, s1, jpf, pq, qq, viol);
                                 if (accepted) {
                                     s.Fe = rebuild(vf, s0, s1);
                                     s.Jp *= jpf;
@@ -421,7 +428,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                         }
                     }
                     if (act) {
-                        if (exact_rm) g2p_return_map(P, s, th, raw[p].dpt, err, vf);
+                        if (exact_rm) g2p_return_map(P, s, th, rw.dpt, err, vf);
                         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
                         bool escaped = false;
                         if (s.cell_updated && !(s.u & kDisabled)) {
@@ -435,7 +442,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pipeline(DevParams P0, St
                             }
                         }
                         // escaped and disabled points leave the pipeline: store all
-                        g2p_store_pipelined(L, i, s, raw[p].u, raw[p].jp, pvx, pvy, C, rec,
+                        g2p_store_pipelined(L, i, s, rw.u, rw.jp, pvx, pvy, C, rec,
                                             S.complete || escaped || (s.u & kDisabled));
                         if (!(s.u & kDisabled) && !escaped) {
                             p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);

@@ -65,9 +65,14 @@ struct MlpDev {
 
 __device__ __forceinline__ int mrow(int j, int lane) { return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + expf(-z)); }
+// SiLU and its derivative from the hardware exp2 and reciprocal (each within
+// a few ulp of FP32, far below the BF16 rounding the activations take next).
+__device__ __forceinline__ float sigmoid_hw(float z) {
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_hw(z); }
 __device__ __forceinline__ float dsilu(float z) {
-    const float s = 1.f / (1.f + expf(-z));
+    const float s = sigmoid_hw(z);
     return s * (1.f + z * (1.f - s));
 }
 
@@ -157,6 +162,56 @@ __device__ __forceinline__ void mlp_forward(const MlpDev &M, const float xs[kMlp
         for (int j = 0; j < 3; j++) y[j] += M.p[kOffB3 + j];
         F.y[bb] = y;
     }
+}
+
+// Inference of a wave's 64 points, one 32-point block at a time (half the
+// live accumulators of mlp_forward); y receives this lane's point's
+// standardised outputs.  Must be called by all 64 lanes.
+__device__ __forceinline__ void mlp_infer(const MlpDev &M, const float xs[kMlpIn], float y[kMlpOut]) {
+    const int lane = threadIdx.x & 63;
+    const f32x16 zero = {};
+    float yb[2][kMlpOut];
+#pragma unroll
+    for (int bb = 0; bb < 2; bb++) {
+        const int src = bb * 32 + (lane & 31);
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = __shfl(xs[i], src, 64);
+        const bf16x4 xop = lane < 32 ? pack4(v[0], v[1], v[2], v[3]) : pack4(0.f, 0.f, 0.f, 0.f);
+        bf16x4 h1[8];
+#pragma unroll
+        for (int ob = 0; ob < 2; ob++) {
+            f32x16 z = mfma(M.pk[(kPkW1A + ob) * 64 + lane], xop, zero);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float a[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) a[i] = silu(z[4 * q + i] + M.p[kOffB1 + ob * 32 + mrow(4 * q + i, lane)]);
+                h1[ob * 4 + q] = pack4(a[0], a[1], a[2], a[3]);
+            }
+        }
+        bf16x4 h2[8];
+#pragma unroll
+        for (int ob = 0; ob < 2; ob++) {
+            f32x16 z = zero;
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) z = mfma(M.pk[(kPkW2A + ob * 8 + kk) * 64 + lane], h1[kk], z);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float a[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) a[i] = silu(z[4 * q + i] + M.p[kOffB2 + ob * 32 + mrow(4 * q + i, lane)]);
+                h2[ob * 4 + q] = pack4(a[0], a[1], a[2], a[3]);
+            }
+        }
+        f32x16 o = zero;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) o = mfma(M.pk[(kPkW3A + kk) * 64 + lane], h2[kk], o);
+#pragma unroll
+        for (int j = 0; j < kMlpOut; j++) yb[bb][j] = __shfl(o[j] + M.p[kOffB3 + j], lane & 31, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < kMlpOut; j++) y[j] = lane < 32 ? yb[0][j] : yb[1][j];
 }
 
 // This lane's point's outputs (standardised) from the forward state.
@@ -503,10 +558,8 @@ __global__ __launch_bounds__(64) void k_mlp_eval(MlpDev M, MlpNorm N, DevParams 
     if (s < n)
         for (int k = 0; k < kMlpIn; k++) xr[k] = X[(size_t)s * kMlpIn + k];
     for (int k = 0; k < kMlpIn; k++) xs[k] = (xr[k] - N.xm[k]) / N.xs[k];
-    MlpFwd F;
-    mlp_forward(M, xs, F);
     float y[kMlpOut];
-    mlp_lane_outputs(F, y);
+    mlp_infer(M, xs, y);
     if (s >= n) return;
     const float coh = 1e5f * expf(xr[3]);
     const float dpt = -coh * P.inv_tan_phi;

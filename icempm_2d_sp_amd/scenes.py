@@ -249,3 +249,30 @@ def batch64_uneven(seed: int = 3003, n_domains: int = 64, min_side: int = 128, m
         out.append(batch_domain(seed * 1000 + d, int(sides[d]), float(rng.uniform(0.25, 0.95)),
                                 float(rng.uniform(0.05, 0.3)), float(rng.uniform(0, 2 * np.pi))))
     return out
+
+
+def batch64_compressed(seed: int = 5005, n_domains: int = 64, min_side: int = 128, max_side: int = 1024,
+                       strain_rate: float = 1.0, wind_pa: float = 1.0) -> list:
+    """Surrogate + batched workload: the batch64_uneven domain set under
+    high-strain-rate forcing.  Each domain's ice starts in uniaxial
+    compression toward its x_max wall, v_x = -strain_rate * (x - x_c) (m/s,
+    with Bp = grad v), under a 1.0 Pa wind stress along +x, so that most
+    points project plastically onto the ridge-capped yield surface every
+    substep (the trial strain increment dt * strain_rate exceeds the yield
+    strain)."""
+    doms = batch64_uneven(seed=seed, n_domains=n_domains, min_side=min_side, max_side=max_side)
+    out = []
+    for sc in doms:
+        soa = sc.soa
+        gx, gy = sc.params.GridXTotal, sc.params.GridYTotal
+        xi = (u32_line(soa, abi.IDX_CELL) & 0xFFFF).astype(np.float64) + soa[abi.IDX_POSX]
+        xc = 0.5 * (gx - 1)
+        soa[abi.IDX_VELX] = (-strain_rate * (xi - xc) * sc.params.cellsize).astype(np.float32)
+        soa[abi.IDX_VELX + 1] = 0.0
+        soa[abi.IDX_BP00] = -strain_rate
+        soa[abi.IDX_BP00 + 1:abi.IDX_BP00 + 4] = 0.0
+        N = gx * gy
+        wind = np.zeros((2, N), np.float32)
+        wind[0] = wind_pa
+        out.append(Scene(sc.params, soa, sc.status, None, wind, sc.name + "c"))
+    return out
