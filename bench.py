@@ -327,8 +327,9 @@ def bench_surrogate(args) -> int:
     t0 = time.perf_counter()
     surr.mlp_generate(1 << 20, seed=3)
     surr.mlp_init(seed=21)
-    loss = np.concatenate([surr.mlp_train(2000, batch=8192, lr=2e-3, seed=5),
-                           surr.mlp_train(2000, batch=8192, lr=4e-4, seed=6)])
+    loss = np.concatenate([surr.mlp_train(8000, batch=8192, lr=2e-3, seed=5),
+                           surr.mlp_train(6000, batch=8192, lr=5e-4, seed=6),
+                           surr.mlp_train(6000, batch=8192, lr=1e-4, seed=7)])
     t_train = time.perf_counter() - t0
     trained, trained_norm = surr.mlp_params(), surr.mlp_norm()
     print(f"[bench] trained in {t_train:.2f}s, loss {loss[0]:.3g} -> {loss[-100:].mean():.3g}", file=sys.stderr,

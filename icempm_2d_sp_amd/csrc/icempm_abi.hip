@@ -114,6 +114,7 @@ struct IceHandle {
     // learned plastic projection
     float *mlp_p = nullptr, *mlp_g = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_loss = nullptr;
     bf16x4 *mlp_pk = nullptr;
+    bf16x8 *mlp_ipk = nullptr;
     uint32_t *mlp_fail = nullptr;
     float *mlp_x = nullptr, *mlp_t = nullptr;   // generated pairs
     uint32_t mlp_n = 0;
@@ -580,7 +581,7 @@ void frames_stale(IceHandle *h) {
 
 SurrArg surr_arg(const IceHandle *h) {
     SurrArg a{};
-    a.M = MlpDev{h->mlp_pk, h->mlp_p};
+    a.M = MlpDev{h->mlp_pk, h->mlp_p, h->mlp_ipk};
     a.N = h->mlp_norm;
     a.tol = h->mlp_tol;
     a.lj0 = (float)h->mlp_samp.logJ_min;
@@ -1028,7 +1029,7 @@ int ice_destroy(IceHandle *h) {
                     (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
                     (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
-                    (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk,
+                    (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk, (void *)h->mlp_ipk,
                     (void *)h->mlp_fail, (void *)h->mlp_x, (void *)h->mlp_t})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
@@ -1482,6 +1483,7 @@ int mlp_alloc(IceHandle *h) {
     HIPCHK(h, hipMalloc(&h->mlp_m, sizeof(float) * kMlpParams));
     HIPCHK(h, hipMalloc(&h->mlp_v, sizeof(float) * kMlpParams));
     HIPCHK(h, hipMalloc(&h->mlp_pk, sizeof(bf16x4) * kPkOps * 64));
+    HIPCHK(h, hipMalloc(&h->mlp_ipk, sizeof(bf16x8) * kIpOps * 64));
     HIPCHK(h, hipMalloc(&h->mlp_fail, sizeof(uint32_t)));
     int rc;
     if ((rc = launch_fill(h, h->mlp_p, kMlpParams))) return rc;
@@ -1505,9 +1507,16 @@ MlpPhys mlp_phys(const IceHandle *h) {
     return m;
 }
 
-int mlp_pack(IceHandle *h) {
+// Master parameters -> packed training operands, and with infer the packed
+// inference operands too.
+int mlp_pack(IceHandle *h, bool infer = true) {
     hipLaunchKernelGGL(k_mlp_pack, dim3(kPkOps), dim3(64), 0, h->stream, (const float *)h->mlp_p, h->mlp_pk);
     LAUNCHCHK(h, "k_mlp_pack");
+    if (infer) {
+        hipLaunchKernelGGL(k_mlp_pack_infer, dim3(kIpOps), dim3(64), 0, h->stream, (const float *)h->mlp_p,
+                           h->mlp_ipk);
+        LAUNCHCHK(h, "k_mlp_pack_infer");
+    }
     return ICE_OK;
 }
 
@@ -1663,7 +1672,7 @@ int ice_mlp_train(IceHandle *h, uint32_t steps, uint32_t batch, float lr, uint64
         h->mlp_loss_cap = steps;
     }
     if (steps && (rc = launch_fill(h, h->mlp_loss, steps))) return rc;
-    const MlpDev M{h->mlp_pk, h->mlp_p};
+    const MlpDev M{h->mlp_pk, h->mlp_p, h->mlp_ipk};
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (uint32_t k = 0; k < steps; k++) {
         if ((rc = launch_fill(h, h->mlp_g, kMlpParams))) return rc;
@@ -1677,7 +1686,7 @@ int ice_mlp_train(IceHandle *h, uint32_t steps, uint32_t batch, float lr, uint64
         hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, h->stream, h->mlp_p,
                            (const float *)h->mlp_g, h->mlp_m, h->mlp_v, lr, b1, b2, eps, c1, c2);
         LAUNCHCHK(h, "k_mlp_adam");
-        if ((rc = mlp_pack(h))) return rc;
+        if ((rc = mlp_pack(h, k + 1 == steps))) return rc;
     }
     if (loss && steps) HIPCHK(h, hipMemcpyAsync(loss, h->mlp_loss, sizeof(float) * steps, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1693,7 +1702,7 @@ int ice_mlp_forward(IceHandle *h, const float *x, uint32_t n, float *y) {
     if ((rc = mlp_upload(h, x, (size_t)n * kMlpIn, &dx))) return rc;
     hipError_t e = hipMalloc(&dy, sizeof(float) * (size_t)n * kMlpOut);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_mlp_forward, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p},
+        hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p, h->mlp_ipk},
                            h->mlp_norm, (const float *)dx, n, dy);
         e = hipGetLastError();
     }
@@ -1714,7 +1723,7 @@ int ice_mlp_grad(IceHandle *h, const float *x, const float *t, uint32_t n, float
     if ((rc = mlp_upload(h, t, (size_t)n * kMlpOut, &dt))) { (void)hipFree(dx); return rc; }
     hipError_t e = hipMalloc(&dl, sizeof(float));
     if (e == hipSuccess && (rc = launch_fill(h, h->mlp_g, kMlpParams)) == ICE_OK && (rc = launch_fill(h, dl, 1)) == ICE_OK) {
-        hipLaunchKernelGGL(k_mlp_train, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p},
+        hipLaunchKernelGGL(k_mlp_train, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p, h->mlp_ipk},
                            h->mlp_norm, (const float *)dx, (const float *)dt, n, n, (uint64_t)0, 0u, 1, h->mlp_g, dl);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(grad, h->mlp_g, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, h->stream);
@@ -1737,7 +1746,7 @@ int ice_mlp_eval(IceHandle *h, const float *x, uint32_t n, float tol, float *out
     if ((rc = mlp_upload(h, x, (size_t)n * kMlpIn, &dx))) return rc;
     hipError_t e = hipMalloc(&dout, sizeof(float) * (size_t)n * 6);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_mlp_eval, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p},
+        hipLaunchKernelGGL(k_mlp_eval, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream, MlpDev{h->mlp_pk, h->mlp_p, h->mlp_ipk},
                            h->mlp_norm, h->dp, mlp_phys(h), (const float *)dx, n, tol, dout);
         e = hipGetLastError();
     }

@@ -11,12 +11,23 @@
 // log(Jp_inv' / Jp_inv).  Features and targets are standardised with fixed
 // per-component means and scales (MlpNorm).
 //
+// Targets are log-magnitudes, log(-log J') and log D' (log J' < 0 and D' > 0
+// on the ridge-capped plastic branch), so that the network's absolute error
+// is a relative error of p' and q'.
+//
 // Layout: every layer is computed transposed, Z^T = W H^T, one wave per 64
 // points (two 32-point column blocks).  The D operand of a 32x32 block (lane
 // l holds column l % 32 and rows 8 (j / 4) + 4 (l / 32) + j % 4 in element j)
-// is exactly the B operand of the next layer's K chunks (chunk q = j / 4,
-// element i = j % 4), so the forward and the input-gradient backward pass
-// stay in registers.  Weight gradients go through LDS transposes.
+// is exactly the B operand of the next layer's K chunks, so the forward and
+// the input-gradient backward pass stay in registers.  Training uses
+// v_mfma_f32_32x32x8_bf16 (chunk q = j / 4, element i = j % 4); weight
+// gradients go through LDS transposes.  Inference uses
+// v_mfma_f32_32x32x16_bf16: K chunk c of 16 takes elements 8 (c % 2) .. + 7
+// of output block c / 2, and the weights are packed in that K order.  Its
+// layer 1 is split-precision: x = x_hi + x_lo and W1 = W1_hi + W1_lo in BF16,
+// one MFMA computes W1_hi x_hi + W1_hi x_lo + W1_lo x_hi + b1_hi + b1_lo (the
+// inputs keep about 16 bits); the layer-2 bias is one more MFMA against a
+// constant operand.
 #pragma once
 #include "point_ops.hpp"
 
@@ -34,7 +45,15 @@ constexpr int kPkW2T = 26;       // [ib 2][kk 8]           A = W2^T (rows in, k 
 constexpr int kPkW3T = 42;       // [ib 2]                 A = W3^T (rows in, k out padded to 8)
 constexpr int kPkOps = 44;
 
+// Packed BF16 operands of inference, in units of 64 lanes x 8 bf16
+constexpr int kIpW1 = 0;    // [ob 2]       A = (W1_hi | W1_hi | W1_lo | b1_hi, b1_lo) against (x_hi | x_lo | x_hi | 1, 1)
+constexpr int kIpW2 = 2;    // [ob 2][c 4]  A = W2 in inference K order
+constexpr int kIpB2 = 10;   // [ob 2]       A = (b2_hi, b2_lo) against (1, 1)
+constexpr int kIpW3 = 12;   // [c 4]        A = W3 (rows out padded to 32) in inference K order
+constexpr int kIpOps = 16;
+
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // FP32 -> BF16, round to nearest even (NaN kept quiet).
@@ -59,9 +78,20 @@ struct MlpNorm {
 };
 
 struct MlpDev {
-    const bf16x4 *pk;   // kPkOps * 64 packed operands
-    const float *p;     // kMlpParams FP32 master parameters (biases are read from here)
+    const bf16x4 *pk;    // kPkOps * 64 packed training operands
+    const float *p;      // kMlpParams FP32 master parameters (training biases, b3)
+    const bf16x8 *ipk;   // kIpOps * 64 packed inference operands
 };
+
+// Hidden unit behind element i of inference K chunk c in lane half hk.
+__host__ __device__ __forceinline__ int ihid(int c, int hk, int i) {
+    return 32 * (c >> 1) + 8 * (2 * (c & 1) + (i >> 2)) + 4 * hk + (i & 3);
+}
+
+__device__ __forceinline__ __bf16 to_bf16(float f) { return __builtin_bit_cast(__bf16, bf16_bits(f)); }
+// BF16 head and tail of an FP32 value: hi + lo carries about 16 significant bits.
+__device__ __forceinline__ float bf16_hi(float f) { return bf16_float(bf16_bits(f)); }
+__device__ __forceinline__ float bf16_lo(float f) { return bf16_float(bf16_bits(f - bf16_hi(f))); }
 
 __device__ __forceinline__ int mrow(int j, int lane) { return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3); }
 
@@ -78,6 +108,9 @@ __device__ __forceinline__ float dsilu(float z) {
 
 __device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
@@ -164,65 +197,56 @@ __device__ __forceinline__ void mlp_forward(const MlpDev &M, const float xs[kMlp
     }
 }
 
-// Inference of a wave's 64 points, one 32-point block at a time (half the
-// live accumulators of mlp_forward); y receives this lane's point's
-// standardised outputs.  Must be called by all 64 lanes.
+// Inference of a wave's 64 points, one 32-point block at a time, on
+// v_mfma_f32_32x32x16_bf16 with the inference operands; y receives this
+// lane's point's standardised outputs.  Must be called by all 64 lanes.
 __device__ __forceinline__ void mlp_infer(const MlpDev &M, const float xs[kMlpIn], float y[kMlpOut]) {
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, hk = lane >> 5;
     const f32x16 zero = {};
+    const __bf16 one = to_bf16(1.f), nil = to_bf16(0.f);
+    bf16x8 ones;
+#pragma unroll
+    for (int i = 0; i < 8; i++) ones[i] = (hk == 0 && i < 2) ? one : nil;
     float yb[2][kMlpOut];
 #pragma unroll
     for (int bb = 0; bb < 2; bb++) {
         const int src = bb * 32 + (lane & 31);
-        float v[4];
+        bf16x8 xop;
 #pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = __shfl(xs[i], src, 64);
-        const bf16x4 xop = lane < 32 ? pack4(v[0], v[1], v[2], v[3]) : pack4(0.f, 0.f, 0.f, 0.f);
-        bf16x4 h1[8];
-#pragma unroll
-        for (int ob = 0; ob < 2; ob++) {
-            f32x16 z = mfma(M.pk[(kPkW1A + ob) * 64 + lane], xop, zero);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                float a[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++) a[i] = silu(z[4 * q + i] + M.p[kOffB1 + ob * 32 + mrow(4 * q + i, lane)]);
-                h1[ob * 4 + q] = pack4(a[0], a[1], a[2], a[3]);
-            }
+        for (int i = 0; i < 4; i++) {
+            const float v = __shfl(xs[i], src, 64);
+            const float hi = bf16_hi(v);
+            xop[i] = to_bf16(hi);
+            xop[4 + i] = hk == 0 ? to_bf16(v - hi) : (i < 2 ? one : nil);
         }
-        bf16x4 h2[8];
+        bf16x8 h1[4];
 #pragma unroll
         for (int ob = 0; ob < 2; ob++) {
-            f32x16 z = zero;
+            const f32x16 z = mfma16(M.ipk[(kIpW1 + ob) * 64 + lane], xop, zero);
 #pragma unroll
-            for (int kk = 0; kk < 8; kk++) z = mfma(M.pk[(kPkW2A + ob * 8 + kk) * 64 + lane], h1[kk], z);
+            for (int hf = 0; hf < 2; hf++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                float a[4];
+                for (int i = 0; i < 8; i++) h1[2 * ob + hf][i] = (__bf16)silu(z[8 * hf + i]);
+        }
+        bf16x8 h2[4];
 #pragma unroll
-                for (int i = 0; i < 4; i++) a[i] = silu(z[4 * q + i] + M.p[kOffB2 + ob * 32 + mrow(4 * q + i, lane)]);
-                h2[ob * 4 + q] = pack4(a[0], a[1], a[2], a[3]);
-            }
+        for (int ob = 0; ob < 2; ob++) {
+            f32x16 z = mfma16(M.ipk[(kIpB2 + ob) * 64 + lane], ones, zero);
+#pragma unroll
+            for (int c = 0; c < 4; c++) z = mfma16(M.ipk[(kIpW2 + ob * 4 + c) * 64 + lane], h1[c], z);
+#pragma unroll
+            for (int hf = 0; hf < 2; hf++)
+#pragma unroll
+                for (int i = 0; i < 8; i++) h2[2 * ob + hf][i] = (__bf16)silu(z[8 * hf + i]);
         }
         f32x16 o = zero;
 #pragma unroll
-        for (int kk = 0; kk < 8; kk++) o = mfma(M.pk[(kPkW3A + kk) * 64 + lane], h2[kk], o);
+        for (int c = 0; c < 4; c++) o = mfma16(M.ipk[(kIpW3 + c) * 64 + lane], h2[c], o);
 #pragma unroll
         for (int j = 0; j < kMlpOut; j++) yb[bb][j] = __shfl(o[j] + M.p[kOffB3 + j], lane & 31, 64);
     }
 #pragma unroll
     for (int j = 0; j < kMlpOut; j++) y[j] = lane < 32 ? yb[0][j] : yb[1][j];
-}
-
-// This lane's point's outputs (standardised) from the forward state.
-__device__ __forceinline__ void mlp_lane_outputs(const MlpFwd &F, float y[kMlpOut]) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int j = 0; j < kMlpOut; j++) {
-        const float a = __shfl(F.y[0][j], lane & 31, 64);
-        const float b = __shfl(F.y[1][j], lane & 31, 64);
-        y[j] = lane < 32 ? a : b;
-    }
 }
 
 // ---- packing, training, evaluation kernels ----------------------------------
@@ -254,6 +278,42 @@ __global__ __launch_bounds__(64) void k_mlp_pack(const float *__restrict__ p, bf
         v[i] = w;
     }
     pk[op * 64 + lane] = pack4(v[0], v[1], v[2], v[3]);
+}
+
+// FP32 master parameters -> BF16 inference operands (one thread per operand
+// lane; element i of lane l is K index 8 (l / 32) + i).
+__global__ __launch_bounds__(64) void k_mlp_pack_infer(const float *__restrict__ p, bf16x8 *__restrict__ ipk) {
+    const int op = blockIdx.x, lane = threadIdx.x;
+    const int r = lane & 31, hk = lane >> 5;
+    bf16x8 v;
+    for (int i = 0; i < 8; i++) {
+        const int k = 8 * hk + i;
+        float w = 0.f;
+        if (op < kIpW2) {                        // W1 split, b1 split: rows out
+            const int out = op * 32 + r;
+            if (k < 12) {
+                const float a = p[kOffW1 + out * kMlpIn + (k & 3)];
+                w = k < 8 ? bf16_hi(a) : bf16_lo(a);
+            } else if (k < 14) {
+                const float b = p[kOffB1 + out];
+                w = k == 12 ? bf16_hi(b) : bf16_lo(b);
+            }
+        } else if (op < kIpB2) {                 // W2: rows out, K chunk c
+            const int ob = (op - kIpW2) >> 2, c = (op - kIpW2) & 3;
+            w = p[kOffW2 + (ob * 32 + r) * kMlpHid + ihid(c, hk, i)];
+        } else if (op < kIpW3) {                 // b2 split at K 0, 1
+            const int ob = op - kIpB2;
+            if (k < 2) {
+                const float b = p[kOffB2 + ob * 32 + r];
+                w = k == 0 ? bf16_hi(b) : bf16_lo(b);
+            }
+        } else {                                  // W3: rows out (< 3), K chunk c
+            const int c = op - kIpW3;
+            if (r < kMlpOut) w = p[kOffW3 + r * kMlpHid + ihid(c, hk, i)];
+        }
+        v[i] = to_bf16(w);
+    }
+    ipk[op * 64 + lane] = v;
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -325,7 +385,7 @@ __global__ __launch_bounds__(256) void k_mlp_pairs(MlpPhys M, MlpSampling S, uin
             const double jp = S.jp0 * exp(log(S.jp1 / S.jp0) * r2);
             const double gauss = sqrt(-2.0 * log(g1)) * cos(6.283185307179586 * g2);
             const double coh = fmin(1e5 * exp(0.5 * gauss), 1e6);
-            ok = exact_projection_d(logJ, D, jp, coh, M, true, y);
+            ok = exact_projection_d(logJ, D, jp, coh, M, true, y) && y[0] < 0.0 && y[1] > 0.0;
             x[0] = logJ;
             x[1] = D;
             x[2] = jp;
@@ -333,7 +393,9 @@ __global__ __launch_bounds__(256) void k_mlp_pairs(MlpPhys M, MlpSampling S, uin
         }
         if (!ok) atomicAdd(n_failed, 1u);
         for (int k = 0; k < kMlpIn; k++) X[(size_t)i * kMlpIn + k] = (float)x[k];
-        for (int k = 0; k < kMlpOut; k++) T[(size_t)i * kMlpOut + k] = ok ? (float)y[k] : 0.f;
+        // targets: log(-log J'), log D', log(Jp' / Jp)
+        const double t[kMlpOut] = {ok ? log(-y[0]) : 0.0, ok ? log(y[1]) : 0.0, ok ? y[2] : 0.0};
+        for (int k = 0; k < kMlpOut; k++) T[(size_t)i * kMlpOut + k] = (float)t[k];
     }
 }
 
@@ -509,18 +571,16 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ p, const f
     p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
-// Forward pass over n rows of raw features: standardised outputs into Y.
-__global__ __launch_bounds__(64) void k_mlp_forward(MlpDev M, MlpNorm N, const float *__restrict__ X, uint32_t n,
-                                                    float *__restrict__ Y) {
+// Inference over n rows of raw features: standardised outputs into Y.
+__global__ __launch_bounds__(64) void k_mlp_infer(MlpDev M, MlpNorm N, const float *__restrict__ X, uint32_t n,
+                                                  float *__restrict__ Y) {
     const int lane = threadIdx.x;
     const uint32_t s = blockIdx.x * 64 + lane;
     float xs[kMlpIn] = {0.f, 0.f, 0.f, 0.f};
     if (s < n)
         for (int k = 0; k < kMlpIn; k++) xs[k] = (X[(size_t)s * kMlpIn + k] - N.xm[k]) / N.xs[k];
-    MlpFwd F;
-    mlp_forward(M, xs, F);
     float y[kMlpOut];
-    mlp_lane_outputs(F, y);
+    mlp_infer(M, xs, y);
     if (s < n)
         for (int k = 0; k < kMlpOut; k++) Y[(size_t)s * kMlpOut + k] = y[k];
 }
@@ -532,8 +592,8 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev M, MlpNorm N, const f
 __device__ __forceinline__ bool mlp_decode(const MlpNorm &N, const float y[kMlpOut], float jp, float dpt, float th,
                                            const DevParams &P, float tol, float &s0, float &s1, float &jpf,
                                            float &p_out, float &q_out, float &viol) {
-    const float lj = N.ym[0] + N.ys[0] * y[0];
-    const float D = N.ym[1] + N.ys[1] * y[1];
+    const float lj = -expf(N.ym[0] + N.ys[0] * y[0]);
+    const float D = expf(N.ym[1] + N.ys[1] * y[1]);
     const float dj = N.ym[2] + N.ys[2] * y[2];
     p_out = -P.half_kappa * expm1f(2.f * lj);
     q_out = 2.f * P.mu * sinhf(D);

@@ -262,12 +262,16 @@ def _norm_vec(norm) -> np.ndarray:
                                 np.float64)
 
 
-def mlp_forward(params, norm, x, bf16: bool = True) -> np.ndarray:
-    """Standardised outputs [n][3] of the surrogate network for raw features x [n][4]."""
+def mlp_forward(params, norm, x, bf16: bool = True, exact_layer1: bool = False) -> np.ndarray:
+    """Standardised outputs [n][3] of the surrogate network for raw features x [n][4].
+
+    bf16: MFMA operands rounded to BF16 (the training kernels' forward pass);
+    with exact_layer1 the first layer is not rounded (the deployed inference)."""
     p = np.ascontiguousarray(params, np.float64)
     xx = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
     y = np.zeros((xx.shape[0], 3))
-    load().orc_mlp_forward(_p(p), _p(_norm_vec(norm)), _p(xx), xx.shape[0], int(bf16), _p(y))
+    mode = (2 if exact_layer1 else 1) if bf16 else 0
+    load().orc_mlp_forward(_p(p), _p(_norm_vec(norm)), _p(xx), xx.shape[0], mode, _p(y))
     return y
 
 

@@ -3,12 +3,15 @@
 // with no reference counterpart): 4 -> 64 -> 64 -> 3 MLP, SiLU, forward pass,
 // mean-squared-error loss and its gradient by the chain rule.
 //
-// With bf16 = true it rounds, like the device, every MFMA operand to BF16
-// (round to nearest even): weights, the standardised features, the hidden
-// activations, and the backward deltas dY, dZ2, dZ1; sums are FP64 here (FP32
-// on the device).  Bias gradients are sums of the rounded deltas, as on the
-// device.  With bf16 = false it is the plain FP64 network, which finite
-// differences pin.
+// Mode 1 (training) rounds, like the device's training kernels, every MFMA
+// operand to BF16 (round to nearest even): weights, the standardised
+// features, the hidden activations, and the backward deltas dY, dZ2, dZ1;
+// sums are FP64 here (FP32 on the device).  Bias gradients are sums of the
+// rounded deltas, as on the device.  Mode 2 (inference) is the device's
+// deployed forward pass: layer 1 unrounded (the device splits its operands
+// into BF16 head and tail, about 16 significant bits), layers 2 and 3 with
+// BF16 operands.  Mode 0 is the plain FP64 network, which finite differences
+// pin.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -46,12 +49,13 @@ struct Trace {
     double x[IN], z1[HID], h1[HID], z2[HID], h2[HID], y[OUT];
 };
 
-inline void forward(const double *p, const double xs[IN], bool b16, Trace &T) {
-    auto r = [&](double v) { return b16 ? bf16(v) : v; };
-    for (int k = 0; k < IN; k++) T.x[k] = r(xs[k]);
+inline void forward(const double *p, const double xs[IN], int mode, Trace &T) {
+    auto r = [&](double v) { return mode != 0 ? bf16(v) : v; };
+    auto r1 = [&](double v) { return mode == 1 ? bf16(v) : v; };   // layer 1
+    for (int k = 0; k < IN; k++) T.x[k] = r1(xs[k]);
     for (int o = 0; o < HID; o++) {
         double z = 0;
-        for (int k = 0; k < IN; k++) z += r(p[W1 + o * IN + k]) * T.x[k];
+        for (int k = 0; k < IN; k++) z += r1(p[W1 + o * IN + k]) * T.x[k];
         T.z1[o] = z + p[B1 + o];
         T.h1[o] = r(silu(T.z1[o]));
     }
@@ -78,7 +82,7 @@ inline double loss_grad(const double *p, const Norm &N, const double *x_raw, con
         double xs[IN], ts[OUT];
         for (int k = 0; k < IN; k++) xs[k] = (x_raw[s * IN + k] - N.xm[k]) / N.xs[k];
         for (int k = 0; k < OUT; k++) ts[k] = (t_raw[s * OUT + k] - N.ym[k]) / N.ys[k];
-        forward(p, xs, b16, T);
+        forward(p, xs, b16 ? 1 : 0, T);
         double dy[OUT];
         for (int o = 0; o < OUT; o++) {
             const double e = T.y[o] - ts[o];

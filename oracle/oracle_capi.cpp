@@ -274,15 +274,16 @@ FUNCS(double, f64)
 FUNCS(float, f32)
 
 // ---- surrogate network (mlp_oracle.hpp) ----
-// x raw [n][4]; norm = {xm[4], xs[4], ym[3], ys[3]}; y standardised outputs [n][3].
-int orc_mlp_forward(const double *params, const double *norm, const double *x, int n, int bf16, double *y) {
+// x raw [n][4]; norm = {xm[4], xs[4], ym[3], ys[3]}; y standardised outputs [n][3];
+// mode 0 FP64, 1 BF16 operands (training), 2 BF16 operands past layer 1 (inference).
+int orc_mlp_forward(const double *params, const double *norm, const double *x, int n, int mode, double *y) {
     orc_mlp::Norm N;
     std::memcpy(&N, norm, sizeof(N));
     orc_mlp::Trace T;
     for (int s = 0; s < n; s++) {
         double xs[orc_mlp::IN];
         for (int k = 0; k < orc_mlp::IN; k++) xs[k] = (x[s * orc_mlp::IN + k] - N.xm[k]) / N.xs[k];
-        orc_mlp::forward(params, xs, bf16 != 0, T);
+        orc_mlp::forward(params, xs, mode, T);
         for (int k = 0; k < orc_mlp::OUT; k++) y[s * orc_mlp::OUT + k] = T.y[k];
     }
     return 0;

@@ -7,8 +7,9 @@ deployment in the pipelined substep with the exact fallback.
 
 Tolerances: forward outputs within 2e-3 of the output scale and gradients
 within 3e-2 (norm-wise) of the oracle's (BF16 operands, FP32 vs FP64 sums: a
-rounding tie can flip one BF16 operand); training targets within 1e-6 of the
-FP64 oracle's (FP32 storage)."""
+rounding tie can flip one BF16 operand); training targets (log(-log J'),
+log D', log(Jp'/Jp)) within 1e-6 relative of the FP64 oracle's (FP32
+storage)."""
 import numpy as np
 import pytest
 
@@ -41,16 +42,21 @@ def sim():
 
 
 def test_forward_matches_oracle(sim):
+    """The deployed inference (32x32x16 MFMAs, split-precision layer 1)
+    against the oracle's inference mode; the all-BF16 network is further
+    away (its rounded inputs and first-layer weights)."""
     _print("test_forward_matches_oracle")
     sim.mlp_init(seed=11)
     p = sim.mlp_params().astype(np.float64)
     nrm = norm_dict(sim.mlp_norm())
     x = sim.pairs[0][:1000]
     y_gpu = sim.mlp_forward(x)
-    y_orc = oracle.mlp_forward(p, nrm, x.astype(np.float64), bf16=True)
+    y_orc = oracle.mlp_forward(p, nrm, x.astype(np.float64), bf16=True, exact_layer1=True)
+    y_b16 = oracle.mlp_forward(p, nrm, x.astype(np.float64), bf16=True)
     scale = np.abs(y_orc).max()
     err = np.abs(y_gpu - y_orc).max() / scale
-    _print("forward max err / scale", err)
+    err_b16 = np.abs(y_gpu - y_b16).max() / scale
+    _print("forward max err / scale", err, "vs the all-BF16 network", err_b16)
     assert err < 2e-3
 
 
@@ -91,11 +97,11 @@ def test_pairs_match_oracle_return_map(sim):
                                                           dp_threshold=-coh / tan_phi)
         assert case == 3, k
         a, b = Fn[0, 0], Fn[1, 1]
-        ref[k] = [np.log(a * b), np.log(a / b), np.log(jpn / jp)]
+        ref[k] = [np.log(-np.log(a * b)), np.log(np.log(a / b)), np.log(jpn / jp)]
     # the features were stored in FP32 after the targets were computed from
-    # the FP64 draws: allow FP32 rounding of the inputs (1e-9 absolute in log J)
-    atol = np.array([1e-12, 1e-10, 2e-9])
-    bad = np.abs(t - ref) / (1e-5 * np.abs(ref) + atol)
+    # the FP64 draws: allow FP32 rounding of the inputs and of the stored targets
+    atol = np.array([5e-7, 5e-7, 2e-9])
+    bad = np.abs(t - ref) / (1e-6 * np.abs(ref) + atol)
     k = int(np.argmax(bad.max(axis=1)))
     _print("worst", bad.max(), "at", k, "x", x[k], "t", t[k], "ref", ref[k])
     assert bad.max() <= 1.0

@@ -93,3 +93,24 @@ def test_bf16_rounding_is_round_to_nearest_even():
         p[4480] = v                     # W3[0][0]
         y = oracle.mlp_forward(p, norm, np.zeros((1, 4)), bf16=True)[0, 0]
         assert y == pytest.approx(e * h2, rel=1e-15, abs=0), v
+
+
+def test_inference_mode_rounds_layers_two_and_three_only():
+    """Mode 2 (the deployed inference) leaves layer 1 unrounded and rounds the
+    operands of layers 2 and 3 to BF16: checked against numpy with the
+    rounding placed there, and it differs from both other modes."""
+    rng = np.random.default_rng(8)
+    p = random_params(rng, 0.4)
+    x, _ = random_rows(rng, 11)
+    W1 = p[0:256].reshape(64, 4); b1 = p[256:320]
+    W2 = numpy_bf16(p[320:4416]).reshape(64, 64); b2 = p[4416:4480]
+    W3 = numpy_bf16(p[4480:4672]).reshape(3, 64); b3 = p[4672:4675]
+    xs = (x - NORM["xm"]) / NORM["xs"]
+    silu = lambda z: z / (1.0 + np.exp(-z))
+    h1 = numpy_bf16(silu(xs @ W1.T + b1))
+    h2 = numpy_bf16(silu(h1 @ W2.T + b2))
+    want = h2 @ W3.T + b3
+    got = oracle.mlp_forward(p, NORM, x, bf16=True, exact_layer1=True)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    assert np.abs(got - oracle.mlp_forward(p, NORM, x, bf16=True)).max() > 1e-6
+    assert np.abs(got - oracle.mlp_forward(p, NORM, x, bf16=False)).max() > 1e-6
