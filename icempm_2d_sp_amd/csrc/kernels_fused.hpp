@@ -17,11 +17,14 @@
 // in LDS; then each cell lane sums the terms of its own points into a
 // register stencil, writes it back to LDS, and each node the round touches
 // gathers the stencils of its (up to) nine cells in a fixed order: plain LDS
-// adds, four barriers per batch of staged points.  Points that changed cell
-// since the sort add their
-// stencils with LDS atomics.  Points that change cell a second time leave
-// the tile pass (kEscaped + kFresh) and the off-tile kernel does their P2G,
-// with global atomics into the frame of the tile that now holds them.
+// adds, four barriers per batch of staged points.  A point that changed cell
+// since the sort (one cell at most) is still summed by the lane of its sorted
+// cell: the part of its stencil inside that lane's 3 x 3 goes into the
+// registers with shifted weights, the rest (three nodes for a move along an
+// axis, five for a diagonal one) into the LDS frame with atomics.  Points that
+// change cell a second time leave the tile pass (kEscaped + kFresh) and the
+// off-tile kernel does their P2G, with global atomics into the frame of the
+// tile that now holds them.
 #pragma once
 #include "tile_common.hpp"
 #include "mlp.hpp"
@@ -104,31 +107,52 @@ __device__ __forceinline__ void p2g_terms(const DevParams &P, float x, float y, 
     q.s11 = (q.mass * Bp.a11 - k * t11) * P.h;
 }
 
-// A point's whole stencil into the LDS frame (origin fx0, fy0) with atomics.
+// A staged point whose cell (cx + sx, cy + sy) differs from the sorted cell
+// (cx, cy) of the lane that sums it: the nodes of its stencil outside the
+// lane's 3 x 3 go into the LDS frame (lane's cell at frame column lx, row ly)
+// with atomics; then t, wx, wy are re-expressed from the lane's cell (weights
+// of the lane's 3 x 3 nodes, zero where the stencil does not reach them).
+// Returns false, flagging it, for a point further than one cell away.
 template <int TW, int TH>
-__device__ __forceinline__ void frame_scatter_lds(const P2GPoint &q, int fx0, int fy0, float *acc, uint32_t &err) {
+__device__ __forceinline__ bool moved_stencil(P2GPoint &t, int sx, int sy, int lx, int ly, float wx[3], float wy[3],
+                                              float *acc, uint32_t &err) {
     using Fr = Frame<TW, TH>;
-    const int lx0 = q.cx - 1 - fx0, ly0 = q.cy - 1 - fy0;
-    if (lx0 < 0 || ly0 < 0 || lx0 + 2 >= Fr::W || ly0 + 2 >= Fr::H) {
+    if (sx < -1 || sx > 1 || sy < -1 || sy > 1) {
         err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
-        return;
+        return false;
     }
-    float wx[3], wy[3];
-    weights(q.x, wx);
-    weights(q.y, wy);
+    // the stencil's column at x offset 2 sx and its row at y offset 2 sy
+    // (new-cell stencil indices ra, rb) lie outside the lane's 3 x 3
+    const int ra = sx > 0 ? 2 : 0, rb = sy > 0 ? 2 : 0;
+    for (int k = 0; k < 6; k++) {
+        const bool col = k < 3;
+        const int a = col ? ra : k - 3, b = col ? k : rb;
+        if (col ? sx == 0 : (sy == 0 || (sx != 0 && a == ra))) continue;   // inside, or the corner again
+        const float w = (col ? (ra ? wx[2] : wx[0]) : (a == 0 ? wx[0] : a == 1 ? wx[1] : wx[2])) *
+                        (col ? (b == 0 ? wy[0] : b == 1 ? wy[1] : wy[2]) : (rb ? wy[2] : wy[0]));
+        float dm, dpx, dpy;
+        p2g_contrib(t, a - 1, b - 1, w, dm, dpx, dpy);
+        if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+        if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+        const int o = (lx + a - 1 + sx) * Fr::LP + (ly + b - 1 + sy);
+        __hip_atomic_fetch_add(&acc[o], dm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&acc[Fr::NN + o], dpx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&acc[2 * Fr::NN + o], dpy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    float sw[3], sh[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 3; a++) {
+        sw[a] = sx == 0 ? wx[a] : (sx > 0 ? (a > 0 ? wx[a - 1] : 0.f) : (a < 2 ? wx[a + 1] : 0.f));
+        sh[a] = sy == 0 ? wy[a] : (sy > 0 ? (a > 0 ? wy[a - 1] : 0.f) : (a < 2 ? wy[a + 1] : 0.f));
+    }
 #pragma unroll
-        for (int b = 0; b < 3; b++) {
-            float dm, dpx, dpy;
-            p2g_contrib(q, a - 1, b - 1, wx[a] * wy[b], dm, dpx, dpy);
-            if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-            if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
-            const int o = (lx0 + a) * Fr::LP + (ly0 + b);
-            __hip_atomic_fetch_add(&acc[o], dm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(&acc[Fr::NN + o], dpx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(&acc[2 * Fr::NN + o], dpy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+    for (int a = 0; a < 3; a++) {
+        wx[a] = sw[a];
+        wy[a] = sh[a];
+    }
+    t.x += (float)sx;
+    t.y += (float)sy;
+    return true;
 }
 
 // A point's whole stencil into the global frame of the tile holding its cell.
@@ -162,7 +186,7 @@ __device__ __forceinline__ void frame_scatter_global(const P2GPoint &q, const Do
 #endif
 constexpr int kPasses = ICE_PIPE_PASSES;
 constexpr int kStage = kPasses * kTileThreads;
-constexpr int kStageFields = 9;
+constexpr int kStageFields = 9;   // x, y, mass, mvx, mvy, s00, s01, s10, s11 (+ the cell, 16 bits)
 // After a batch is summed, the staging area holds the 27 stencil sums of each
 // of the round's 256 cells.
 static_assert(kStageFields * kStage >= 27 * kTileThreads, "staging area too small for the cell stencils");
@@ -187,9 +211,13 @@ __device__ __forceinline__ void load_g2p_raw(const Lines &L, const DevParams &P,
     r.dpt = dp_threshold(P, L, i);
 }
 
-// A point's P2G terms into staging slot jj (mass 0 marks an empty slot).
-__device__ __forceinline__ void stage_terms(float (*s_st)[kStage], uint32_t jj, bool stage, const P2GPoint &q) {
+// A point's P2G terms into staging slot jj (mass 0 marks an empty slot), and
+// its cell in frame coordinates (column | row << 8; 0xffff if outside).
+__device__ __forceinline__ void stage_terms(float (*s_st)[kStage], uint16_t *s_cw, uint32_t jj, bool stage,
+                                            const P2GPoint &q, int fx0, int fy0) {
     if (stage) {
+        const uint32_t lx = (uint32_t)(q.cx - fx0), ly = (uint32_t)(q.cy - fy0);
+        s_cw[jj] = (lx < 256u && ly < 256u) ? (uint16_t)(lx | (ly << 8)) : (uint16_t)0xffffu;
         s_st[0][jj] = q.x;
         s_st[1][jj] = q.y;
         s_st[2][jj] = q.mass;
@@ -245,6 +273,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
     __shared__ float s_acc[3 * Fr::NN];
     __shared__ float2 s_v[FUSED ? Fr::N : 1];
     __shared__ float s_st[kStageFields][kStage];
+    __shared__ uint16_t s_cw[kStage];
     __shared__ uint32_t s_rb[RC];
     __shared__ uint32_t s_pre[RC + 1];
 
@@ -446,11 +475,10 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                                             S.complete || escaped || (s.u & kDisabled));
                         if (!(s.u & kDisabled) && !escaped) {
                             p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);
-                            if (s.u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
-                            else stage = true;
+                            stage = true;
                         }
                     }
-                    stage_terms(s_st, jj, stage, q);
+                    stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
                 }
             } else {
 #pragma unroll
@@ -464,15 +492,14 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                         if (!(u & (kDisabled | kEscaped))) {
                             load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
                             if (!stencil_inside(q.cx, q.cy, P)) err |= ICE_ERR_POINT_LEFT_AREA;
-                            else if (u & kMoved) frame_scatter_lds<TW, TH>(q, fx0, fy0, s_acc, err);
                             else stage = true;
                         }
                     }
-                    stage_terms(s_st, jj, stage, q);
+                    stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
                 }
             }
             __syncthreads();
-            // each cell lane sums its own staged points
+            // each cell lane sums the staged points sorted into its cell
             float am[3][3], ax[3][3], ay[3][3];
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -496,6 +523,10 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                 float wx[3], wy[3];
                 weights(t.x, wx);
                 weights(t.y, wy);
+                const uint32_t pc = s_cw[k];
+                const int sx = (int)(pc & 0xffu) - (cx - fx0), sy = (int)(pc >> 8) - (cy - fy0);
+                if ((sx | sy) != 0 && !moved_stencil<TW, TH>(t, sx, sy, cx - fx0, cy - fy0, wx, wy, s_acc, err))
+                    continue;
 #pragma unroll
                 for (int a = 0; a < 3; a++)
 #pragma unroll
