@@ -89,7 +89,7 @@ struct IceHandle {
              *s_order = nullptr, *s_bsum = nullptr;
     size_t s_bsum_len = 0;
     // tiles
-    int tile_w = 32, tile_h = 32, band = 4;
+    int tile_w = 16, tile_h = 32, band = 4;
     FixScale fix{};
     float max_thickness = 1.f;
     // phase state

@@ -200,6 +200,10 @@ __device__ __forceinline__ void mlp_forward(const MlpDev &M, const float xs[kMlp
 // Inference of a wave's 64 points, one 32-point block at a time, on
 // v_mfma_f32_32x32x16_bf16 with the inference operands; y receives this
 // lane's point's standardised outputs.  Must be called by all 64 lanes.
+// Register-lean for use inside the tile pass: the blocks run one after the
+// other (the operand loads are issued per block, not hoisted out of the
+// block loop), and layer 3 consumes each pair of layer-2 chunks as soon as
+// it exists.
 __device__ __forceinline__ void mlp_infer(const MlpDev &M, const float xs[kMlpIn], float y[kMlpOut]) {
     const int lane = threadIdx.x & 63, hk = lane >> 5;
     const f32x16 zero = {};
@@ -207,9 +211,10 @@ __device__ __forceinline__ void mlp_infer(const MlpDev &M, const float xs[kMlpIn
     bf16x8 ones;
 #pragma unroll
     for (int i = 0; i < 8; i++) ones[i] = (hk == 0 && i < 2) ? one : nil;
-    float yb[2][kMlpOut];
-#pragma unroll
+#pragma unroll 1
     for (int bb = 0; bb < 2; bb++) {
+        const bf16x8 *ops = M.ipk;
+        __asm__ volatile("" : "+s"(ops));   // keeps this block's operand loads inside the loop
         const int src = bb * 32 + (lane & 31);
         bf16x8 xop;
 #pragma unroll
@@ -222,31 +227,32 @@ __device__ __forceinline__ void mlp_infer(const MlpDev &M, const float xs[kMlpIn
         bf16x8 h1[4];
 #pragma unroll
         for (int ob = 0; ob < 2; ob++) {
-            const f32x16 z = mfma16(M.ipk[(kIpW1 + ob) * 64 + lane], xop, zero);
+            const f32x16 z = mfma16(ops[(kIpW1 + ob) * 64 + lane], xop, zero);
 #pragma unroll
             for (int hf = 0; hf < 2; hf++)
 #pragma unroll
                 for (int i = 0; i < 8; i++) h1[2 * ob + hf][i] = (__bf16)silu(z[8 * hf + i]);
         }
-        bf16x8 h2[4];
-#pragma unroll
-        for (int ob = 0; ob < 2; ob++) {
-            f32x16 z = mfma16(M.ipk[(kIpB2 + ob) * 64 + lane], ones, zero);
-#pragma unroll
-            for (int c = 0; c < 4; c++) z = mfma16(M.ipk[(kIpW2 + ob * 4 + c) * 64 + lane], h1[c], z);
-#pragma unroll
-            for (int hf = 0; hf < 2; hf++)
-#pragma unroll
-                for (int i = 0; i < 8; i++) h2[2 * ob + hf][i] = (__bf16)silu(z[8 * hf + i]);
-        }
         f32x16 o = zero;
 #pragma unroll
-        for (int c = 0; c < 4; c++) o = mfma16(M.ipk[(kIpW3 + c) * 64 + lane], h2[c], o);
+        for (int ob = 0; ob < 2; ob++) {
+            f32x16 z = mfma16(ops[(kIpB2 + ob) * 64 + lane], ones, zero);
 #pragma unroll
-        for (int j = 0; j < kMlpOut; j++) yb[bb][j] = __shfl(o[j] + M.p[kOffB3 + j], lane & 31, 64);
+            for (int c = 0; c < 4; c++) z = mfma16(ops[(kIpW2 + ob * 4 + c) * 64 + lane], h1[c], z);
+#pragma unroll
+            for (int hf = 0; hf < 2; hf++) {
+                bf16x8 h2;
+#pragma unroll
+                for (int i = 0; i < 8; i++) h2[i] = (__bf16)silu(z[8 * hf + i]);
+                o = mfma16(ops[(kIpW3 + 2 * ob + hf) * 64 + lane], h2, o);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kMlpOut; j++) {
+            const float v = __shfl(o[j] + M.p[kOffB3 + j], lane & 31, 64);
+            if (hk == bb) y[j] = v;
+        }
     }
-#pragma unroll
-    for (int j = 0; j < kMlpOut; j++) y[j] = lane < 32 ? yb[0][j] : yb[1][j];
 }
 
 // ---- packing, training, evaluation kernels ----------------------------------

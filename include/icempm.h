@@ -184,8 +184,8 @@ int ice_enable_timings(IceHandle *h, int enable);
 int ice_get_timings(IceHandle *h, double out[8]);
 
 /* Tile of grid nodes per workgroup of the sorted path (width along x, height
- * along y).  Supported: 8x8, 8x16, 16x8, 16x16, 16x32, 32x16, 32x32, 32x64,
- * 64x32, 64x64.  No reference counterpart (the reference's only launch knobs
+ * along y).  Supported: 8x8, 8x16, 16x8, 16x16, 16x32 (default), 32x16, 32x32,
+ * 32x64, 64x32, 64x64.  No reference counterpart (the reference's only launch knobs
  * are tpb_P2G/Upd/G2P, parameters_sim.cpp:111-113). */
 int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
 /* Tuning switches of the sorted path (no reference counterpart): "band"
