@@ -294,7 +294,8 @@ def bench_batch(args) -> int:
 
 def bench_surrogate(args) -> int:
     """batch64_compressed: 64 uneven domains in uniaxial compression at 1/s
-    under 1 Pa wind (most points project plastically every substep).  Times
+    against the fixed wall at x_max under 1 Pa wind (most points project
+    plastically every substep).  Times
     (a) plain batched substeps (exact return map), (b) batched substeps with
     the learned plastic projection and the exact fallback at
     --surrogate-tol, and (c) the plain single-domain runs one after another.
@@ -383,7 +384,7 @@ def bench_surrogate(args) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 state, bf16 MLP",
         "data": "synthetic",
         "config": {"workload": "batch64_compressed: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, "
-                               "uniaxial compression 1/s toward x_max, 1.0 Pa wind along +x, 4 points/cell",
+                               "uniaxial compression at 1/s against the fixed wall at x_max, 1.0 Pa wind along +x, 4 points/cell",
                    "points": n_active, "surrogate_tol": tol},
         "plain_batched": {"value": n_active * steps / wall_plain, "ms_per_step": wall_plain * 1000 / steps},
         "plain_sequential": {"value": n_active * steps / wall_seq, "ms_per_step": wall_seq * 1000 / steps},

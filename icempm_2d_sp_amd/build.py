@@ -16,9 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libicempm.so")
 
-SOURCES = ["icempm_abi.hip"]
-HEADERS = ["device_math.hpp", "point_ops.hpp", "kernels_naive.hpp", "kernels_sort.hpp", "tile_common.hpp",
-           "kernels_p2g.hpp", "kernels_g2p.hpp"]
+SOURCES = ["icempm_abi.hip"]   # includes every kernel header of csrc/
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -255,19 +255,19 @@ def batch64_compressed(seed: int = 5005, n_domains: int = 64, min_side: int = 12
                        strain_rate: float = 1.0, wind_pa: float = 1.0) -> list:
     """Surrogate + batched workload: the batch64_uneven domain set under
     high-strain-rate forcing.  Each domain's ice starts in uniaxial
-    compression toward its x_max wall, v_x = -strain_rate * (x - x_c) (m/s,
-    with Bp = grad v), under a 1.0 Pa wind stress along +x, so that most
-    points project plastically onto the ridge-capped yield surface every
-    substep (the trial strain increment dt * strain_rate exceeds the yield
-    strain)."""
+    compression against the fixed wall at x_max (the first wall node column,
+    x_w = gx - 2): v_x = strain_rate * (x_w - x) (m/s, with Bp = grad v),
+    under a 1.0 Pa wind stress along +x, so that most points project
+    plastically onto the ridge-capped yield surface every substep (the trial
+    strain increment dt * strain_rate exceeds the yield strain)."""
     doms = batch64_uneven(seed=seed, n_domains=n_domains, min_side=min_side, max_side=max_side)
     out = []
     for sc in doms:
         soa = sc.soa
         gx, gy = sc.params.GridXTotal, sc.params.GridYTotal
         xi = (u32_line(soa, abi.IDX_CELL) & 0xFFFF).astype(np.float64) + soa[abi.IDX_POSX]
-        xc = 0.5 * (gx - 1)
-        soa[abi.IDX_VELX] = (-strain_rate * (xi - xc) * sc.params.cellsize).astype(np.float32)
+        x_wall = gx - 2.0
+        soa[abi.IDX_VELX] = (strain_rate * (x_wall - xi) * sc.params.cellsize).astype(np.float32)
         soa[abi.IDX_VELX + 1] = 0.0
         soa[abi.IDX_BP00] = -strain_rate
         soa[abi.IDX_BP00 + 1:abi.IDX_BP00 + 4] = 0.0

@@ -246,7 +246,8 @@ int ice_stream_bench(int device, size_t bytes, int reps, int mode, double *ms_pe
  * A 4 -> 64 -> 64 -> 3 MLP with SiLU, BF16 weights and activations on MFMA,
  * FP32 accumulation.  Features of a trial state: log J, D = log(s0/s1),
  * Jp_inv, log(c / 1e5) (s0 >= s1 the singular values of the elastic
- * predictor, c the cohesion); targets: the projected log J', D' and
+ * predictor, c the cohesion); targets: log(-log J'), log D' of the projected
+ * state (log J' < 0, D' > 0 on the ridge-capped branch) and
  * log(Jp_inv' / Jp_inv).  Parameters: W1[64][4], b1[64], W2[64][64], b2[64],
  * W3[3][64], b3[3] (ICE_MLP_PARAMS floats). */
 #define ICE_MLP_PARAMS 4675
@@ -278,7 +279,8 @@ int ice_mlp_init_params(IceHandle *h, uint64_t seed);
  * pairs (mean squared error of the standardised targets); loss per step into
  * `loss` when not NULL. */
 int ice_mlp_train(IceHandle *h, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *loss);
-/* Forward pass over n raw feature rows x [n][4]: standardised outputs y [n][3]. */
+/* Forward pass over n raw feature rows x [n][4]: standardised outputs y [n][3],
+ * by the deployed inference (BF16 layers 2 and 3, split-precision layer 1). */
 int ice_mlp_forward(IceHandle *h, const float *x, uint32_t n, float *y);
 /* Gradient of the mean squared error over the n rows (x raw, t raw) into
  * grad (ICE_MLP_PARAMS), loss into *loss; parameters unchanged. */
