@@ -75,6 +75,15 @@ class IceMlpSampling(C.Structure):
                 ("jp_min", C.c_double), ("jp_max", C.c_double)]
 
 
+MAX_REGIONS, FRAME_FIELDS = 100, 10
+FRAME_FIELD_NAMES = ("density", "mass", "vx", "vy", "r", "g", "b", "Jpinv", "P", "Q")
+
+
+class IceFrameArrays(C.Structure):
+    _fields_ = [("count", C.c_void_p), ("fields", C.c_void_p), ("rgb", C.c_void_p), ("mass_mask", C.c_void_p),
+                ("forces_per_region", C.c_void_p)]
+
+
 _P = C.c_void_p
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
@@ -100,6 +109,8 @@ SIGNATURES = {
     "ice_clear_force_accumulator": (C.c_int, [_P]),
     "ice_download": (C.c_int, [_P, _P, C.c_size_t, C.c_int, _P, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
     "ice_download_grid": (C.c_int, [_P, _P, _P]),
+    "ice_upload_point_colors": (C.c_int, [_P, _P, C.c_uint32]),
+    "ice_prepare_frame_arrays": (C.c_int, [_P, C.c_int, C.POINTER(IceFrameArrays)]),
     "ice_synchronize": (C.c_int, [_P]),
     "ice_sort": (C.c_int, [_P, C.POINTER(C.c_uint32), _P]),
     "ice_reset_timings": (C.c_int, [_P]),

@@ -26,6 +26,7 @@
 #include "kernels_p2g.hpp"
 #include "kernels_g2p.hpp"
 #include "kernels_fused.hpp"
+#include "kernels_frame.hpp"
 
 using namespace ice;
 
@@ -115,6 +116,8 @@ struct IceHandle {
     float *mlp_p = nullptr, *mlp_g = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_loss = nullptr;
     bf16x4 *mlp_pk = nullptr;
     bf16x8 *mlp_ipk = nullptr;
+    uint32_t *colors = nullptr;   // point colours by original point index
+    uint32_t ncolors = 0;
     uint32_t *mlp_fail = nullptr;
     float *mlp_x = nullptr, *mlp_t = nullptr;   // generated pairs
     uint32_t mlp_n = 0;
@@ -1029,7 +1032,7 @@ int ice_destroy(IceHandle *h) {
                     (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
                     (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
-                    (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk, (void *)h->mlp_ipk,
+                    (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk, (void *)h->mlp_ipk, (void *)h->colors,
                     (void *)h->mlp_fail, (void *)h->mlp_x, (void *)h->mlp_t})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
@@ -1230,6 +1233,79 @@ int ice_download_domain(IceHandle *h, int d, float *soa, size_t host_pitch, int 
                         uint32_t *error_code, int32_t *disabled_count) {
     if (!h) return ICE_E_ARG;
     return download_d(h, d, soa, host_pitch, n_lines, forces, error_code, disabled_count);
+}
+
+int ice_upload_point_colors(IceHandle *h, const uint32_t *rgb, uint32_t n) {
+    if (!h || (n && !rgb)) return ICE_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->colors) (void)hipFree(h->colors);
+    h->colors = nullptr;
+    h->ncolors = 0;
+    if (n) {
+        HIPCHK(h, hipMalloc(&h->colors, sizeof(uint32_t) * n));
+        HIPCHK(h, hipMemcpy(h->colors, rgb, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+        h->ncolors = n;
+    }
+    return ICE_OK;
+}
+
+int ice_prepare_frame_arrays(IceHandle *h, int elapsed_cycles, IceFrameArrays *out) {
+    if (!h || !out) return ICE_E_ARG;
+    if (h->ndom() != 1) { h->err = "frame arrays of a single-domain handle"; return ICE_E_STATE; }
+    if (h->naive()) { h->err = "frame arrays need the sorted path"; return ICE_E_STATE; }
+    if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = device_sort(h, 0, false, nullptr);   // every point in its cell's range
+    if (rc) return rc;
+    const Domain &D = h->dom[0];
+    const size_t N = D.N;
+    // one buffer: count [N] u8, mask [N] u8, rgb [3N] u8, fields [10][N] f32, partials, regions
+    const int rblocks = 256;
+    const size_t o_mask = round_up(N, 256), o_rgb = o_mask + round_up(N, 256), o_fields = o_rgb + round_up(3 * N, 256);
+    const size_t o_part = o_fields + round_up(sizeof(float) * kFrameFields * N, 256);
+    const size_t o_reg = o_part + sizeof(double) * 2 * kMaxRegions * rblocks;
+    char *buf = nullptr;
+    HIPCHK(h, hipMalloc(&buf, o_reg + sizeof(float) * 2 * kMaxRegions));
+    FrameOut fo{out->count ? (uint8_t *)buf : nullptr, out->fields ? (float *)(buf + o_fields) : nullptr,
+                out->rgb ? (uint8_t *)(buf + o_rgb) : nullptr, out->mass_mask ? (uint8_t *)(buf + o_mask) : nullptr};
+    DomainDesc dd{};
+    dd.gx = D.gx;
+    dd.gy = D.gy;
+    hipError_t e = hipSuccess;
+    if (fo.count || fo.fields || fo.rgb || fo.mask) {
+        hipLaunchKernelGGL(k_frame_fields, dim3((unsigned)std::min<size_t>(blocks_for(N, 256), 8192)), dim3(256), 0,
+                           h->stream, dd, h->dlines(0), (const uint32_t *)(h->cs + D.cs_base),
+                           (const uint8_t *)(h->status + D.node_base), (const uint32_t *)h->colors, h->ncolors,
+                           (float)h->prm.ParticleMass, fo);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && out->forces_per_region) {
+        const double scale = elapsed_cycles > 0 ? 1.0 / (elapsed_cycles * h->prm.InitialTimeStep) : 1.0;
+        hipLaunchKernelGGL(k_region_forces, dim3(rblocks), dim3(256), 0, h->stream, dd,
+                           (const float *)h->dgline(G_FX, 0), (const float *)h->dgline(G_FY, 0),
+                           (const uint8_t *)(h->status + D.node_base), scale, (double *)(buf + o_part));
+        e = hipGetLastError();
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_region_sum, dim3(1), dim3(256), 0, h->stream, (const double *)(buf + o_part), rblocks,
+                               (float *)(buf + o_reg));
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out->forces_per_region, buf + o_reg, sizeof(float) * 2 * kMaxRegions,
+                               hipMemcpyDeviceToHost, h->stream);
+    }
+    if (e == hipSuccess && out->count) e = hipMemcpyAsync(out->count, buf, N, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && out->mass_mask)
+        e = hipMemcpyAsync(out->mass_mask, buf + o_mask, N, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && out->rgb) e = hipMemcpyAsync(out->rgb, buf + o_rgb, 3 * N, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && out->fields)
+        e = hipMemcpyAsync(out->fields, buf + o_fields, sizeof(float) * kFrameFields * N, hipMemcpyDeviceToHost,
+                           h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) { h->err = std::string("ice_prepare_frame_arrays: ") + hipGetErrorString(e); return ICE_E_HIP; }
+    return ICE_OK;
 }
 
 int ice_download_grid(IceHandle *h, float *mpp, float *velocity) {

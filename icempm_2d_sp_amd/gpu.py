@@ -438,6 +438,29 @@ class Simulation(SurrogateMixin):
         self._chk(self.lib.ice_download_grid(self.h, abi.ptr(mpp), abi.ptr(vel)), "ice_download_grid")
         return mpp, vel
 
+    def upload_point_colors(self, rgb):
+        """Point colours 0xRRGGBB by original point index (point_colors_rgb)."""
+        rgb = np.ascontiguousarray(rgb, np.uint32).reshape(-1)
+        self._chk(self.lib.ice_upload_point_colors(self.h, abi.ptr(rgb), rgb.shape[0]), "ice_upload_point_colors")
+
+    def prepare_frame_arrays(self, elapsed_cycles: int = 0,
+                             want=("count", "fields", "rgb", "mass_mask", "forces_per_region")) -> dict:
+        """SnapshotManager::PrepareFrameArrays (snapshotmanager.cpp:623-765) on
+        the device: the ten fields [N] by name (abi.FRAME_FIELD_NAMES; Jpinv
+        is Jp_inv - 1), count, rgb [N][3], mass_mask, forces_per_region
+        [100][2] (those named in `want`).  Sorts the points by cell first."""
+        N = self.params.GridXTotal * self.params.GridYTotal
+        count = np.zeros(N, np.uint8) if "count" in want else None
+        fields = np.zeros((abi.FRAME_FIELDS, N), np.float32) if "fields" in want else None
+        rgb = np.zeros((N, 3), np.uint8) if "rgb" in want else None
+        mask = np.zeros(N, np.uint8) if "mass_mask" in want else None
+        fpr = np.zeros((abi.MAX_REGIONS, 2), np.float32) if "forces_per_region" in want else None
+        fa = abi.IceFrameArrays(abi.ptr(count), abi.ptr(fields), abi.ptr(rgb), abi.ptr(mask), abi.ptr(fpr))
+        self._chk(self.lib.ice_prepare_frame_arrays(self.h, elapsed_cycles, C.byref(fa)), "ice_prepare_frame_arrays")
+        out = {name: fields[k] for k, name in enumerate(abi.FRAME_FIELD_NAMES)} if fields is not None else {}
+        out.update(count=count, rgb=rgb, mass_mask=mask, forces_per_region=fpr)
+        return out
+
     def sort(self, want_perm: bool = False):
         n0 = self.info().n_points
         perm = np.zeros(max(n0, 1), np.uint32) if want_perm else None

@@ -159,6 +159,30 @@ int ice_clear_force_accumulator(IceHandle *h);                      /* gpu_parti
  * error word and the disabled count.  Synchronous. */
 int ice_download(IceHandle *h, float *soa, size_t host_pitch, int n_lines,
                  float *boundary_forces, uint32_t *error_code, int32_t *disabled_count);
+/* ---- frame arrays: SnapshotManager::PrepareFrameArrays
+ * (data_manager/snapshotmanager.cpp:623-765) on the device.  Point colours
+ * (0xRRGGBB per original point index, the reference's point_colors_rgb) are
+ * uploaded once; n may be 0 (every point black). */
+int ice_upload_point_colors(IceHandle *h, const uint32_t *rgb, uint32_t n);
+#define ICE_MAX_REGIONS 100          /* SimParams::MAX_REGIONS, parameters_sim.h:46 */
+#define ICE_FRAME_FIELDS 10          /* point density, mass, vx, vy, r, g, b, Jp_inv - 1, P, Q */
+typedef struct IceFrameArrays {      /* host arrays, any may be NULL (skipped) */
+    uint8_t *count;                  /* [gx*gy] points per cell, modulo 256 (the reference's uint8 counter) */
+    float *fields;                   /* [ICE_FRAME_FIELDS][gx*gy]: vis_point_density, vis_mass, vis_vx, vis_vy,
+                                        vis_r, vis_g, vis_b, vis_Jpinv, vis_P, vis_Q after normalisation */
+    uint8_t *rgb;                    /* [gx*gy][3] colour blended with the water colour */
+    uint8_t *mass_mask;              /* [gx*gy] point density > 0.1 */
+    float *forces_per_region;        /* [ICE_MAX_REGIONS][2] boundary forces summed per region */
+} IceFrameArrays;
+/* The frame arrays of a single-domain handle from its current points and
+ * force accumulator; boundary forces are divided by elapsed_cycles *
+ * InitialTimeStep (as transfer_from_device leaves them,
+ * gpu_implementation5.cpp:219-221) when elapsed_cycles > 0.  Sorts the points
+ * by cell first (the device order changes; ICE_IDX_POINT identifies them).
+ * Regions past ICE_MAX_REGIONS are skipped (the reference writes past its
+ * array).  Synchronous. */
+int ice_prepare_frame_arrays(IceHandle *h, int elapsed_cycles, IceFrameArrays *out);
+
 /* Grid fields after the last substep: [3][gx*gy] m, px, py as P2G left them
  * (momentum), or NULL; [2][gx*gy] node velocities after the update, or NULL. */
 int ice_download_grid(IceHandle *h, float *mpp, float *velocity);

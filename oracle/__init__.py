@@ -61,6 +61,8 @@ def load():
         "orc_sort": (C.c_int, [P, C.POINTER(C.c_uint32), P]),
         "orc_params_derive": (None, [PP]),
         "orc_mlp_forward": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
+        "orc_frame_arrays": (C.c_int, [P, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_double, P, P, C.c_int, P,
+                                       C.c_double, P, P, P, P, P]),
         "orc_mlp_grad": (C.c_double, [P, P, P, P, C.c_int, C.c_int, P]),
     }
     for suf, T in (("f64", C.c_double), ("f32", C.c_float)):
@@ -283,3 +285,30 @@ def mlp_grad(params, norm, x, t, bf16: bool = True):
     g = np.zeros(p.shape[0])
     loss = load().orc_mlp_grad(_p(p), _p(_norm_vec(norm)), _p(xx), _p(tt), xx.shape[0], int(bf16), _p(g))
     return float(loss), g
+
+
+FRAME_FIELDS = ("density", "mass", "vx", "vy", "r", "g", "b", "Jpinv", "P", "Q")
+
+
+def frame_arrays(soa, gx: int, gy: int, particle_mass: float, status, colors=None, forces=None,
+                 fscale: float = 1.0) -> dict:
+    """SnapshotManager::PrepareFrameArrays (snapshotmanager.cpp:623-765) on a
+    host SoA soa [lines][n] (float32, the device's layout): the ten fields
+    [gx*gy] (Jpinv already minus 1), count (uint8), rgb [gx*gy][3], mass_mask,
+    forces_per_region [100][2] (forces [2][gx*gy] scaled by fscale)."""
+    soa = np.ascontiguousarray(soa, np.float32)
+    n = soa.shape[1]
+    N = gx * gy
+    status = np.ascontiguousarray(status, np.uint8).reshape(-1)
+    colors = np.zeros(0, np.uint32) if colors is None else np.ascontiguousarray(colors, np.uint32)
+    forces = np.zeros((2, N), np.float32) if forces is None else np.ascontiguousarray(forces, np.float32)
+    count = np.zeros(N, np.uint8)
+    fields = np.zeros((10, N), np.float32)
+    rgb = np.zeros((N, 3), np.uint8)
+    mask = np.zeros(N, np.uint8)
+    fpr = np.zeros((100, 2), np.float32)
+    load().orc_frame_arrays(_p(soa), n, n, gx, gy, float(particle_mass), _p(status), _p(colors), colors.shape[0],
+                            _p(forces), float(fscale), _p(count), _p(fields), _p(rgb), _p(mask), _p(fpr))
+    out = {name: fields[k] for k, name in enumerate(FRAME_FIELDS)}
+    out.update(count=count, rgb=rgb, mass_mask=mask, forces_per_region=fpr)
+    return out

@@ -2,6 +2,7 @@
 // CPU baseline.  TEST INFRASTRUCTURE ONLY: never linked into the product.
 #include "mpm_oracle.hpp"
 #include "mlp_oracle.hpp"
+#include "frame_oracle.hpp"
 #include <chrono>
 
 using namespace orc;
@@ -295,6 +296,16 @@ double orc_mlp_grad(const double *params, const double *norm, const double *x, c
     std::memcpy(&N, norm, sizeof(N));
     std::memset(grad, 0, sizeof(double) * orc_mlp::NPARAM);
     return orc_mlp::loss_grad(params, N, x, t, n, bf16 != 0, grad);
+}
+
+// ---- frame arrays (frame_oracle.hpp) ----
+// soa [>= 19][pitch] float lines; forces [2][gx*gy]; fields [10][gx*gy]; any output may be NULL.
+int orc_frame_arrays(const float *soa, size_t pitch, int n, int gx, int gy, double particle_mass,
+                     const uint8_t *status, const uint32_t *colors, int ncolors, const float *forces, double fscale,
+                     uint8_t *count, float *fields, uint8_t *rgb, uint8_t *mask, float *fpr) {
+    orc_frame::prepare(soa, pitch, n, gx, gy, particle_mass, status, colors, ncolors, forces, fscale, count, fields,
+                       rgb, mask, fpr);
+    return 0;
 }
 
 }  // extern "C"
