@@ -25,6 +25,11 @@ HIPCC_FLAGS = [
     "-fPIC",
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fused only inside one source expression, never across
+    # statements after inlining: a device function then rounds the same way in
+    # every kernel that inlines it (deterministic mode depends on it: a point's
+    # G2P on the tile and the off-tile path must give the same bits)
+    "-ffp-contract=on",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-command-line-argument",
@@ -53,7 +58,33 @@ def build_library(verbose: bool = False) -> str:
         print(res.stderr, file=sys.stderr)
     os.replace(tmp, LIB)
     build_driver(verbose)
+    build_io(verbose)
     return LIB
+
+
+IO_LIB = os.path.join(LIBDIR, "libicempm_io.so")
+HDF5_ROOT = os.environ.get("ICE_HDF5_ROOT", "/opt/conda")
+
+
+def build_io(verbose: bool = False) -> str | None:
+    """libicempm_io.so: the reference's HDF5 snapshot and flow-field formats
+    (host C++ against the HDF5 1.10 C++ API).  Skipped, with a message, when
+    the HDF5 headers are not found under ICE_HDF5_ROOT (default /opt/conda)."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "H5Cpp.h")):
+        print(f"libicempm_io.so not built: no HDF5 C++ headers under {inc}", file=sys.stderr)
+        return None
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", inc,
+           os.path.join(CSRC, "h5io.cpp"), "-L", lib, "-lhdf5_cpp", "-lhdf5", f"-Wl,-rpath,{lib}",
+           "-o", IO_LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"libicempm_io.so build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    return IO_LIB
 
 
 DRIVER = os.path.join(LIBDIR, "model_driver")

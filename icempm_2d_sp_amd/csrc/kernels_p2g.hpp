@@ -11,9 +11,11 @@
 // LDS atomics in a phase of their own.  Every node of the tile is then stored
 // once: no global atomics and no grid clear.
 //
-// Deterministic mode: the LDS sums are int64 fixed point (order-independent)
-// and the tile kernel also adds the domain's off-tile points, so results are
-// bit-identical for any schedule.  Otherwise k_p2g_offtile adds the off-tile
+// Deterministic mode: every contribution is converted to int64 fixed point
+// on its own and the sums are integer (independent of order and of how points
+// are grouped into cells), and the tile kernel also adds the domain's
+// off-tile points, so results are bit-identical for any schedule and any sort
+// history.  Otherwise k_p2g_offtile adds the off-tile
 // points with float global atomics after the tiles.
 #pragma once
 #include "tile_common.hpp"
@@ -57,16 +59,21 @@ __device__ __forceinline__ void scatter_point(const P2GPoint &q, int x0, int y0,
         }
 }
 
-// Sum the unmoved points of cell (x, y) into the register stencil; returns
-// whether any point contributed and sets *moved when the cell also holds
-// points that changed cell since the sort.
+template <bool DET>
+using StencilAcc = typename std::conditional<DET, long long, float>::type;
+
+// Sum the unmoved points of cell (x, y) into the register stencil (DET: each
+// contribution in fixed point); returns whether any point contributed and
+// sets *moved when the cell also holds points that changed cell since the sort.
+template <bool DET>
 __device__ __forceinline__ bool cell_stencil(const Lines &L, const DevParams &P, uint32_t b, uint32_t e, int x, int y,
-                                             bool ring2, float am[3][3], float ax[3][3], float ay[3][3],
-                                             bool &moved, uint32_t &err) {
+                                             bool ring2, StencilAcc<DET> am[3][3], StencilAcc<DET> ax[3][3],
+                                             StencilAcc<DET> ay[3][3], bool &moved, uint32_t &err,
+                                             const FixScale &fs) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) am[i][j] = ax[i][j] = ay[i][j] = 0.f;
+        for (int j = 0; j < 3; j++) am[i][j] = ax[i][j] = ay[i][j] = 0;
     bool any = false;
     moved = false;
     const bool inside = x >= 1 && y >= 1 && x <= P.gx - 2 && y <= P.gy - 2;
@@ -87,20 +94,30 @@ __device__ __forceinline__ bool cell_stencil(const Lines &L, const DevParams &P,
             for (int bb = 0; bb < 3; bb++) {
                 float dm, dpx, dpy;
                 p2g_contrib(q, a - 1, bb - 1, wx[a] * wy[bb], dm, dpx, dpy);
-                am[a][bb] += dm;
-                ax[a][bb] += dpx;
-                ay[a][bb] += dpy;
+                if constexpr (DET) {   // the conversion scatter_point's contributions take
+                    if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+                    if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+                    am[a][bb] += (long long)(dm * fs.m);
+                    ax[a][bb] += (long long)(dpx * fs.p);
+                    ay[a][bb] += (long long)(dpy * fs.p);
+                } else {
+                    am[a][bb] += dm;
+                    ax[a][bb] += dpx;
+                    ay[a][bb] += dpy;
+                }
             }
         any = true;
     }
-    if (any) {
+    if constexpr (!DET) {
+        if (any) {
 #pragma unroll
-        for (int a = 0; a < 3; a++)
+            for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int bb = 0; bb < 3; bb++) {
-                if (__builtin_isnan(ax[a][bb]) || __builtin_isnan(ay[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-                if (__builtin_isnan(am[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
-            }
+                for (int bb = 0; bb < 3; bb++) {
+                    if (__builtin_isnan(ax[a][bb]) || __builtin_isnan(ay[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+                    if (__builtin_isnan(am[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+                }
+        }
     }
     return any;
 }
@@ -158,9 +175,9 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const 
             e = cs[cell + 1];
         }
         const bool ring2 = (c == 0 || c == RW - 1 || r == 0 || r == RH - 1);
-        float am[3][3], ax[3][3], ay[3][3];
+        StencilAcc<DET> am[3][3], ax[3][3], ay[3][3];
         bool moved = false;
-        const bool any = valid && cell_stencil(L, P, b, e, x, y, ring2, am, ax, ay, moved, err);
+        const bool any = valid && cell_stencil<DET>(L, P, b, e, x, y, ring2, am, ax, ay, moved, err, fs);
         if constexpr (DET) {
             if (any) {
 #pragma unroll
@@ -168,8 +185,12 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const 
 #pragma unroll
                     for (int bb = 0; bb < 3; bb++) {
                         const int lx = x + a - 1 - x0, ly = y + bb - 1 - y0;
-                        if ((unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH)
-                            lds_add3<DET, NN>(s_acc, lx * LP + ly, am[a][bb], ax[a][bb], ay[a][bb], fs);
+                        if ((unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH) {
+                            const int o = lx * LP + ly;
+                            atomicAdd(&s_acc[o], (unsigned long long)am[a][bb]);
+                            atomicAdd(&s_acc[NN + o], (unsigned long long)ax[a][bb]);
+                            atomicAdd(&s_acc[2 * NN + o], (unsigned long long)ay[a][bb]);
+                        }
                     }
             }
             if (moved) scatter_moved<TW, TH, LP, DET>(L, P, b, e, x0, y0, s_acc, fs, err);
