@@ -1,5 +1,6 @@
-"""The C-ABI library loads and exports every entry point include/icempm.h
-declares; the ctypes layouts match the C structs.  No compute calls (no GPU)."""
+"""The C-ABI libraries load and export every entry point include/icempm.h and
+include/icempm_io.h declare; the ctypes layouts match the C structs.  No
+compute calls (no GPU)."""
 import ctypes as C
 import os
 import re
@@ -13,8 +14,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "icempm.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(ice_\w+)\s*\(", text, flags=re.M)))
 
@@ -31,6 +32,17 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert set(declared_functions()) == set(abi.SIGNATURES), \
         "ctypes signatures and header disagree"
+
+
+def test_io_library_exports_every_declared_symbol():
+    from icempm_2d_sp_amd import h5io
+    if not os.path.exists(h5io.LIB_PATH):
+        pytest.skip("libicempm_io.so not built (no HDF5)")
+    names = declared_functions(os.path.join(ROOT, "include", "icempm_io.h"))
+    lib = h5io.load()
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(h5io.SIGNATURES)
 
 
 def test_params_reset_and_derive_without_gpu():
@@ -52,8 +64,10 @@ def test_struct_layout_matches_c():
 #include <stddef.h>
 #include "icempm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu\n", sizeof(IceParams), offsetof(IceParams, flags),
-         offsetof(IceParams, windStressToAccel), sizeof(IceInfo), offsetof(IceInfo, n_tiles));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(IceParams), offsetof(IceParams, flags),
+         offsetof(IceParams, windStressToAccel), sizeof(IceInfo), offsetof(IceInfo, n_tiles),
+         sizeof(IceFrameArrays), offsetof(IceFrameArrays, forces_per_region), sizeof(IceMlpNorm),
+         offsetof(IceMlpNorm, ys), sizeof(IceMlpSampling));
   return 0; }
 '''
     tmp = "/tmp/icempm_layout_probe"
@@ -63,7 +77,9 @@ int main(void) {
     out = subprocess.run([tmp], capture_output=True, text=True, check=True).stdout.split()
     got = [int(v) for v in out]
     exp = [C.sizeof(abi.IceParams), abi.IceParams.flags.offset, abi.IceParams.windStressToAccel.offset,
-           C.sizeof(abi.IceInfo), abi.IceInfo.n_tiles.offset]
+           C.sizeof(abi.IceInfo), abi.IceInfo.n_tiles.offset, C.sizeof(abi.IceFrameArrays),
+           abi.IceFrameArrays.forces_per_region.offset, C.sizeof(abi.IceMlpNorm), abi.IceMlpNorm.ys.offset,
+           C.sizeof(abi.IceMlpSampling)]
     assert got == exp
 
 
