@@ -751,6 +751,11 @@ int alloc_grid(IceHandle *h) {
 
 // (Re)allocate the point buffers and the scratch that depends on capacity.
 int alloc_points(IceHandle *h) {
+    size_t base = 0;
+    for (const auto &D : h->dom) base = round_up(base + std::max<uint32_t>(D.capacity, 1), 64);
+    // kernels address a line slot by a 32-bit byte offset; a rejected
+    // capacity leaves the current allocation as it was
+    if (base >= (1ull << 30)) { h->err = "total point capacity exceeds 2^30 slots"; return ICE_E_CAPACITY; }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     drop_graph(h);
     for (void **p : {(void **)&h->pts, (void **)&h->pts_alt, (void **)&h->offtile, (void **)&h->s_key,
@@ -759,15 +764,13 @@ int alloc_points(IceHandle *h) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
-    size_t base = 0;
+    base = 0;
     for (auto &D : h->dom) {
         D.pt_base = (uint32_t)base;
         D.n = 0;
         D.sorted = false;
         base = round_up(base + std::max<uint32_t>(D.capacity, 1), 64);
     }
-    // kernels address a line slot by a 32-bit byte offset
-    if (base >= (1ull << 30)) { h->err = "total point capacity exceeds 2^30 slots"; return ICE_E_CAPACITY; }
     h->total_capacity = (uint32_t)base;
     h->pitch = base;
     const size_t bytes = sizeof(float) * h->pitch * ICE_N_LINES_MAX;
@@ -1045,8 +1048,11 @@ int ice_allocate(IceHandle *h, uint32_t cap) {
     if (!h) return ICE_E_ARG;
     if (h->batch) { h->err = "capacities of a batch are fixed by ice_create_batch"; return ICE_E_STATE; }
     HIPCHK(h, hipSetDevice(h->device));
+    const uint32_t old = h->dom[0].capacity;
     h->dom[0].capacity = cap;
-    return alloc_points(h);
+    const int rc = alloc_points(h);
+    if (rc == ICE_E_CAPACITY) h->dom[0].capacity = old;
+    return rc;
 }
 
 int ice_upload_points(IceHandle *h, const float *soa, size_t host_pitch, uint32_t n, int n_lines) {
