@@ -378,6 +378,22 @@ def bench_surrogate(args) -> int:
         wall_seq += timed(s1)
         s1.close()
     print(f"[bench] plain sequential {wall_seq:.3f}s", file=sys.stderr, flush=True)
+    # the reference's own launch structure (unsorted points, global atomics,
+    # ICE_FLAG_NAIVE), one domain after another, over 64 substeps
+    naive_steps = 64
+    wall_naive = 0.0
+    for sc in doms:
+        s1 = Simulation(sc, flags=sc.params.flags | abi.FLAG_NAIVE)
+        s1.run(0, 16, graph=True)
+        s1.sync()
+        t0 = time.perf_counter()
+        s1.run(16, naive_steps, graph=True)
+        s1.sync()
+        wall_naive += time.perf_counter() - t0
+        s1.close()
+    ms_naive = wall_naive * 1000 / naive_steps
+    print(f"[bench] plain sequential, reference launch structure: {ms_naive:.3f} ms per substep", file=sys.stderr,
+          flush=True)
     plastic = (n_acc + n_fb) / (n_active * steps)
     outj = {
         "metric": "particle-updates/s per substep, 64 batched uneven domains, surrogate + batched vs plain",
@@ -391,7 +407,12 @@ def bench_surrogate(args) -> int:
                    "points": n_active, "surrogate_tol": tol},
         "plain_batched": {"value": n_active * steps / wall_plain, "ms_per_step": wall_plain * 1000 / steps},
         "plain_sequential": {"value": n_active * steps / wall_seq, "ms_per_step": wall_seq * 1000 / steps},
+        "plain_sequential_reference_launch_structure": {
+            "value": n_active * 1000 / ms_naive, "ms_per_step": ms_naive, "substeps": naive_steps,
+            "note": "unsorted points, P2G with global atomics, separate update and G2P (ICE_FLAG_NAIVE)"},
         "surrogate_batched_over_plain_sequential": wall_seq / wall_surr,
+        "surrogate_batched_over_plain_sequential_reference_launch_structure":
+            ms_naive / (wall_surr * 1000 / steps),
         "surrogate_batched_over_plain_batched": wall_plain / wall_surr,
         "ridge_capped_plastic_fraction_per_substep": plastic,
         "surrogate_accepted_fraction": n_acc / max(n_acc + n_fb, 1),
