@@ -1,0 +1,76 @@
+"""The plain substep at full size (icefield 4096x2048, 31.8M points, the
+benchmark's workload), checked through properties that hold at any size:
+P2G conserves mass and momentum (the weights have unit sum and zero first
+moment, so sum_i m_i = sum_p m_p and sum_i p_i = sum_p m_p v_p), no error
+flag is raised, every point's position is cell-local and its cell inside the
+grid, and a device sort leaves keys non-decreasing and the points a
+bijection of the input.  Tolerances: 2e-5 relative for the conserved sums
+(FP32 frame sums of 31.8M points against FP64 sums of the FP32 states)."""
+import numpy as np
+import pytest
+
+from icempm_2d_sp_amd import abi, scenes
+from icempm_2d_sp_amd.gpu import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def sim():
+    _print("full-size scene 4096x2048")
+    sc = scenes.icefield_4096x2048()
+    s = Simulation(sc)
+    s.scene_n = sc.n
+    yield s
+    s.close()
+
+
+def test_conservation_and_invariants_at_full_size(sim):
+    prm = sim.params
+    dt = prm.InitialTimeStep
+    sim.run(0, 16, graph=True)
+    for k in range(16, 20):
+        sim.substep(k * dt, False)
+    pts, _, err, dis = sim.download()             # state after substep 19
+    sim.substep(20 * dt, True)
+    mpp, _ = sim.grid()                           # the P2G of that state
+    _print("points", pts.shape[1], "error word", err, "disabled", dis)
+    assert err == 0
+    util = pts[abi.IDX_UTILITY].view(np.uint32)
+    live = (util & abi.STATUS_DISABLED) == 0
+    m = prm.ParticleMass * pts[abi.IDX_THICKNESS].astype(np.float64)
+    mass_p = float((m * live).sum())
+    mom_p = [float((m * pts[abi.IDX_VELX + c].astype(np.float64) * live).sum()) for c in range(2)]
+    scale_p = float((m * np.hypot(pts[abi.IDX_VELX], pts[abi.IDX_VELX + 1]) * live).sum())
+    mass_g = float(mpp[0].astype(np.float64).sum())
+    mom_g = [float(mpp[1 + c].astype(np.float64).sum()) for c in range(2)]
+    _print(f"mass points {mass_p:.9e} grid {mass_g:.9e}; momentum points {mom_p} grid {mom_g} (scale {scale_p:.3e})")
+    assert mass_g == pytest.approx(mass_p, rel=2e-5)
+    for c in range(2):
+        assert abs(mom_g[c] - mom_p[c]) <= 2e-5 * scale_p
+    cw = pts[abi.IDX_CELL].view(np.uint32)
+    cx, cy = cw & 0xFFFF, cw >> 16
+    assert (cx[live] < prm.GridXTotal).all() and (cy[live] < prm.GridYTotal).all()
+    for c in range(2):
+        x = pts[abi.IDX_POSX + c][live]
+        assert np.isfinite(x).all() and (np.abs(x) <= 0.5).all()
+
+
+def test_sort_at_full_size(sim):
+    n0 = sim.info().n_points
+    before, _, _, _ = sim.download()
+    ids0 = np.sort(before[abi.IDX_POINT].view(np.uint32)[(before[abi.IDX_UTILITY].view(np.uint32)
+                                                          & abi.STATUS_DISABLED) == 0])
+    perm = sim.sort(want_perm=True)
+    after, _, _, _ = sim.download()
+    n = after.shape[1]
+    _print("sort:", n0, "->", n, "points (disabled dropped)")
+    cw = after[abi.IDX_CELL].view(np.uint32)
+    key = (cw & 0xFFFF).astype(np.int64) * sim.params.GridYTotal + (cw >> 16)
+    assert (np.diff(key) >= 0).all()
+    assert np.array_equal(np.sort(after[abi.IDX_POINT].view(np.uint32)), ids0)
+    assert perm.shape[0] == n and np.unique(perm).shape[0] == n
