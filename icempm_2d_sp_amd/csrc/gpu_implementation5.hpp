@@ -247,6 +247,30 @@ public:
     void record_timings(bool enablePointTransfer) { for (auto &p : partitions) p.record_timings(enablePointTransfer); }
 };
 
+// The per-frame arrays of SnapshotManager (data_manager/snapshotmanager.h:53-64),
+// computed on the device in place of SnapshotManager::PrepareFrameArrays
+// (snapshotmanager.cpp:623-765); call after transfer_from_device with the same
+// elapsed_cycles.
+struct FrameArrays {
+    std::vector<uint8_t> count, rgb, mass_mask;
+    std::vector<float> vis_point_density, vis_mass, vis_vx, vis_vy, vis_r, vis_g, vis_b, vis_Jpinv, vis_P, vis_Q;
+    float forces_per_region[ICE_MAX_REGIONS][2] = {};
+
+    void prepare(GPU_Implementation5 &gpu, int elapsed_cycles) {
+        GPU_Partition &p = gpu.partitions.front();
+        const size_t N = (size_t)gpu.prms->GridXTotal * gpu.prms->GridYTotal;
+        count.assign(N, 0);
+        rgb.assign(3 * N, 0);
+        mass_mask.assign(N, 0);
+        std::vector<float> fields((size_t)ICE_FRAME_FIELDS * N);
+        IceFrameArrays out{count.data(), fields.data(), rgb.data(), mass_mask.data(), &forces_per_region[0][0]};
+        check(ice_prepare_frame_arrays(p.handle, elapsed_cycles, &out), p.handle, "PrepareFrameArrays");
+        std::vector<float> *dst[ICE_FRAME_FIELDS] = {&vis_point_density, &vis_mass, &vis_vx, &vis_vy, &vis_r,
+                                                      &vis_g, &vis_b, &vis_Jpinv, &vis_P, &vis_Q};
+        for (int f = 0; f < ICE_FRAME_FIELDS; f++) dst[f]->assign(fields.begin() + f * N, fields.begin() + (f + 1) * N);
+    }
+};
+
 inline void GPU_Partition::transfer_grid_data_to_device(GPU_Implementation5 *gpu) {
     check(ice_upload_grid(handle, gpu->grid_status_buffer.data(),
                           gpu->grid_boundary_normals.empty() ? nullptr : gpu->grid_boundary_normals.data()),

@@ -1,7 +1,8 @@
 // model_driver.cpp — a Model::Step-style host driver (simulation/model.cpp:7-78)
 // written against the drop-in GPU_Implementation5 shim, unchanged in shape:
 // per frame clear_force_accumulator, then reset_grid / p2g / update_nodes /
-// g2p / record_timings per substep, then transfer_from_device and the timing
+// g2p / record_timings per substep, then transfer_from_device, the frame
+// arrays (SnapshotManager::PrepareFrameArrays, on the device) and the timing
 // table.  Scene: a rectangular ice sheet, 4 points per cell, on a grid with a
 // two-node wall ring, pushed by a uniform wind stress.
 //
@@ -87,6 +88,7 @@ int main(int argc, char **argv) {
 
         int SimulationStep = 0;
         double simulation_time = 0;
+        FrameArrays frame;
         for (int f = 0; f < frames; f++) {
             gpu.reset_timings();
             gpu.clear_force_accumulator();
@@ -114,6 +116,15 @@ int main(int argc, char **argv) {
                         p.timing_10_P2GAndHalo, p.timing_30_updateGrid, p.timing_40_G2P, p.timing_stepTotal,
                         vx / std::max(1u, gpu.hssoa.size));
             if (gpu.error_code) return 2;
+            frame.prepare(gpu, count);
+            size_t ice_nodes = 0;
+            double mass = 0;
+            for (size_t i = 0; i < frame.mass_mask.size(); i++) {
+                ice_nodes += frame.mass_mask[i];
+                mass += frame.vis_mass[i];
+            }
+            std::printf("frame %d arrays: ice nodes %zu, mass on water nodes %.6e, wall force region 1 (%.4e, %.4e)\n",
+                        f, ice_nodes, mass, frame.forces_per_region[1][0], frame.forces_per_region[1][1]);
         }
         gpu.synchronize();
         std::printf("model_driver done\n");

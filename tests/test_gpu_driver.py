@@ -22,4 +22,6 @@ def test_model_driver_runs_frames():
     assert [int(f[1]) for f in frames] == [50, 100, 150]
     v = [float(f[2]) for f in frames]
     assert 0 < v[0] < v[1] < v[2]     # the wind keeps accelerating the sheet
+    arrays = re.findall(r"frame \d+ arrays: ice nodes (\d+), mass on water nodes ([-0-9.e+]+)", res.stdout)
+    assert len(arrays) == 3 and all(int(a[0]) > 0 and float(a[1]) > 0 for a in arrays)
     assert "model_driver done" in res.stdout
