@@ -106,6 +106,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--naive", action="store_true", help="reference launch structure (unsorted, atomics)")
     ap.add_argument("--unfused", action="store_true", help="separate P2G and update + G2P tile kernels")
+    ap.add_argument("--persistent", action="store_true", help="tile passes as persistent launches (work queues)")
     ap.add_argument("--scale", type=float, default=1.0, help="linear domain scale (1.0 = 4096x2048)")
     ap.add_argument("--workload", choices=["icefield", "batch64", "surrogate", "frame"], default="icefield",
                     help="icefield: one 4096x2048 domain (the headline); batch64: 64 uneven domains; "
@@ -152,7 +153,8 @@ def main() -> int:
     tw, th = (int(v) for v in args.tile.split("x"))
     flags = scene.params.flags | (abi.FLAG_NAIVE if args.naive else 0)
     pipelined = not (args.naive or args.unfused)
-    sim = Simulation(scene, flags=flags, tile=(tw, th), options={"pipelined": int(pipelined)})
+    sim = Simulation(scene, flags=flags, tile=(tw, th),
+                     options={"pipelined": int(pipelined), "persistent": int(args.persistent)})
     n_active = sim.info().n_points
     print(f"[bench] uploaded and sorted; warmup {args.warmup}", file=sys.stderr, flush=True)
     sim.run(0, args.warmup, graph=False)
@@ -225,7 +227,8 @@ def main() -> int:
                          "random polygonal ice field (~95%), GRF wind stress (mean 0.2 Pa, std 0.05 Pa, "
                          "corr 200 cells), lognormal cohesion (median 1e5 Pa, sigma 0.5), dt=1e-4 s"),
             "points": n_active, "nodes": nodes, "tile": args.tile,
-            "path": "naive" if args.naive else ("sorted-unfused" if args.unfused else "sorted-pipelined"),
+            "path": "naive" if args.naive else ("sorted-unfused" if args.unfused else
+                                                ("sorted-pipelined-persistent" if args.persistent else "sorted-pipelined")),
             "error_word": err,
         },
         "roofline": {
@@ -257,7 +260,7 @@ def bench_batch(args) -> int:
     print(f"[bench] {len(doms)} domains, {sum(d.n for d in doms)} points, generated in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     steps = max(16, (args.steps // 16) * 16)
-    b = BatchSimulation(doms)
+    b = BatchSimulation(doms, options={"persistent": int(args.persistent)})
     tile, tune_ms = b.autotune(2)
     n_active = b.n_points()
     b.run(0, 16, graph=True)

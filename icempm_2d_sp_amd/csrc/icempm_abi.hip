@@ -108,6 +108,9 @@ struct IceHandle {
     bool frames_valid = false;     // fr[fr_cur] holds the P2G of the current point state
     bool last_fused = false;       // the last substep ran pipelined: its grid is fr[fr_cur ^ 1]
     bool fused_opt = true;
+    bool persistent = false;          // option "persistent": tile passes as persistent launches
+    uint32_t *tile_queue = nullptr;   // their per-XCD tile counters
+    int n_cu = 0;
     // graph
     hipGraphExec_t graph_exec = nullptr;
     int graph_chunk = 0;
@@ -272,7 +275,7 @@ using P2GFn = void (*)(DevParams, const DomainDesc *, const TileRef *, int, Line
 using G2PFn = void (*)(DevParams, StepArg, const DomainDesc *, const TileRef *, int, Lines, const uint32_t *,
                        GridLines, Counters *, uint32_t *);
 using PipeFn = void (*)(DevParams, StepArg, SurrArg, const DomainDesc *, const TileRef *, int, Lines,
-                        const uint32_t *, GridLines, Counters *, uint32_t *, const float *, float *);
+                        const uint32_t *, GridLines, Counters *, uint32_t *, const float *, float *, uint32_t *);
 using OffFn = void (*)(DevParams, StepArg, const DomainDesc *, Lines, GridLines, Counters *, const uint32_t *,
                        const float *, float *, int);
 using ToGridFn = void (*)(const DomainDesc *, int, const float *, GridLines);
@@ -281,15 +284,15 @@ struct TileKernels {
     int w, h;
     P2GFn p2g, p2g_det;
     G2PFn g2p;
-    PipeFn fill, fused, fused_surr;
+    PipeFn fill, fused, fused_surr, fill_persistent, fused_persistent;
     OffFn offtile;
     ToGridFn to_grid;
 };
 
 #define TK(W, H)                                                                                          \
     {W, H, k_p2g_tiles<W, H, false>, k_p2g_tiles<W, H, true>, k_g2p_tiles<W, H>, k_tile_pipeline<W, H, false>, \
-     k_tile_pipeline<W, H, true>, k_tile_pipeline<W, H, true, true>, k_offtile_pipeline<W, H>,               \
-     k_frames_to_grid<W, H>}
+     k_tile_pipeline<W, H, true>, k_tile_pipeline<W, H, true, true>, k_tile_pipeline<W, H, false, false, true>, \
+     k_tile_pipeline<W, H, true, false, true>, k_offtile_pipeline<W, H>, k_frames_to_grid<W, H>}
 const TileKernels kTiles[] = {TK(8, 8),   TK(8, 16),  TK(16, 8),  TK(16, 16), TK(16, 32),
                               TK(32, 16), TK(32, 32), TK(32, 64), TK(64, 32), TK(64, 64)};
 #undef TK
@@ -596,6 +599,25 @@ SurrArg surr_arg(const IceHandle *h) {
     return a;
 }
 
+// Grid and queue of a k_tile_pipeline launch: one workgroup per tile, or
+// (option "persistent") as many workgroups as fit on the device at once, a
+// multiple of 8 (one queue per XCD, each with workgroups), pulling tiles from
+// the per-XCD queues, zeroed here by a kernel (graph-safe).
+int tile_launch_shape(IceHandle *h, PipeFn &fn, PipeFn persistent_fn, unsigned &grid, uint32_t *&queue) {
+    grid = (unsigned)h->ntiles;
+    queue = nullptr;
+    if (!h->persistent || !persistent_fn) return ICE_OK;
+    fn = persistent_fn;
+    int occ = 0;
+    HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, kTileThreads, 0));
+    const unsigned fit = (unsigned)std::max(8, h->n_cu * std::max(occ, 1) / 8 * 8);
+    grid = std::min<unsigned>(fit, (unsigned)round_up((size_t)h->ntiles, 8));
+    int rc = launch_fill_u32(h, h->tile_queue, 8, 0u);
+    if (rc) return rc;
+    queue = h->tile_queue;
+    return ICE_OK;
+}
+
 // Pipeline fill: P2G of the current point state into fr[fr_cur].
 int pipeline_fill(IceHandle *h) {
     const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
@@ -603,9 +625,14 @@ int pipeline_fill(IceHandle *h) {
     const StepArg S0{0.0, 0, nullptr, 0};
     float *F = h->fr[h->fr_cur];
     if (h->ntiles) {
-        hipLaunchKernelGGL(tk->fill, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, S0, SurrArg{},
-                           h->d_desc, h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt,
-                           h->offtile, (const float *)F, F);
+        unsigned grid;
+        uint32_t *queue;
+        PipeFn fn = tk->fill;
+        int rc = tile_launch_shape(h, fn, tk->fill_persistent, grid, queue);
+        if (rc) return rc;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kTileThreads), 0, h->stream, h->dp, S0, SurrArg{}, h->d_desc,
+                           h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt, h->offtile,
+                           (const float *)F, F, queue);
         LAUNCHCHK(h, "k_tile_pipeline (fill)");
     }
     hipLaunchKernelGGL(tk->offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S0, h->d_desc,
@@ -635,9 +662,13 @@ int phase_pipelined(IceHandle *h, const StepArg &S) {
     float *Fn = h->fr[h->fr_cur ^ 1];
     if (h->ntiles) {
         const bool surr = h->mlp_deployed && h->mlp_have_params && h->mlp_have_norm;
-        hipLaunchKernelGGL(surr ? tk->fused_surr : tk->fused, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp,
-                           S, surr ? surr_arg(h) : SurrArg{}, h->d_desc, h->d_tiles, h->ntiles, h->lines(),
-                           (const uint32_t *)h->cs, G, h->cnt, h->offtile, Fc, Fn);
+        PipeFn fn = surr ? tk->fused_surr : tk->fused;
+        unsigned grid;
+        uint32_t *queue;
+        if ((rc = tile_launch_shape(h, fn, surr ? nullptr : tk->fused_persistent, grid, queue))) return rc;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kTileThreads), 0, h->stream, h->dp, S,
+                           surr ? surr_arg(h) : SurrArg{}, h->d_desc, h->d_tiles, h->ntiles, h->lines(),
+                           (const uint32_t *)h->cs, G, h->cnt, h->offtile, Fc, Fn, queue);
         LAUNCHCHK(h, "k_tile_pipeline");
     }
     rec(h, 4);
@@ -1035,7 +1066,7 @@ int ice_destroy(IceHandle *h) {
                     (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
                     (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
-                    (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk, (void *)h->mlp_ipk, (void *)h->colors,
+                    (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk, (void *)h->mlp_ipk, (void *)h->colors, (void *)h->tile_queue,
                     (void *)h->mlp_fail, (void *)h->mlp_x, (void *)h->mlp_t})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
@@ -1124,6 +1155,13 @@ int ice_set_option(IceHandle *h, const char *name, int value) {
         if (value != 0 && value != 1) { h->err = "pipelined must be 0 or 1"; return ICE_E_ARG; }
         h->fused_opt = value != 0;
         frames_stale(h);
+    } else if (std::strcmp(name, "persistent") == 0) {
+        if (value != 0 && value != 1) { h->err = "persistent must be 0 or 1"; return ICE_E_ARG; }
+        HIPCHK(h, hipSetDevice(h->device));
+        if (value && !h->tile_queue) HIPCHK(h, hipMalloc(&h->tile_queue, 8 * sizeof(uint32_t)));
+        if (value && !h->n_cu)
+            HIPCHK(h, hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device));
+        h->persistent = value != 0;
     } else {
         h->err = std::string("unknown option ") + name;
         return ICE_E_ARG;

@@ -243,360 +243,381 @@ struct SurrArg {
     float lj0, lj1, d0, d1, jp0, jp1;   // feature ranges it was trained on
 };
 
-template <int TW, int TH, bool FUSED, bool SURR = false>
+template <int TW, int TH, bool FUSED, bool SURR = false, bool PERSIST = false>
 __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0, StepArg S, SurrArg SA,
                                                                 const DomainDesc *__restrict__ D,
                                                                 const TileRef *__restrict__ T, int ntiles, Lines L0,
                                                                 const uint32_t *__restrict__ cs0, GridLines G0,
                                                                 Counters *cnt0, uint32_t *off0,
-                                                                const float *__restrict__ Fc, float *__restrict__ Fn) {
+                                                                const float *__restrict__ Fc, float *__restrict__ Fn,
+                                                                uint32_t *queue) {
     using Fr = Frame<TW, TH>;
     static_assert(kTileThreads % TH == 0 && TH <= kTileThreads, "a round holds whole tile columns");
     constexpr int RC = kTileThreads / TH;              // tile columns per round
     constexpr int NROUND = (TW + RC - 1) / RC;
-    const TileRef tr = T[xcd_remap(blockIdx.x, ntiles)];
-    const DomainDesc dd = D[tr.d];
-    DevParams P = P0;
-    P.gx = dd.gx;
-    P.gy = dd.gy;
-    P.has_current = dd.has_current;
-    const Lines L{L0.base + dd.pt_base, L0.pitch};
-    const uint32_t *__restrict__ cs = cs0 + dd.cs_base;
-    const size_t nb = dd.node_base;
-    const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
-                      G0.wind_x + nb, G0.wind_y + nb, G0.status + nb};
-    Counters *cnt = cnt0 + tr.d;
-    uint32_t *__restrict__ offtile = off0 + dd.off_base;
-    const int x0 = tr.tx * TW, y0 = tr.ty * TH;
-    const int fx0 = x0 - 2, fy0 = y0 - 2;                // frame origin
-
     __shared__ float s_acc[3 * Fr::NN];
     __shared__ float2 s_v[FUSED ? Fr::N : 1];
     __shared__ float s_st[kStageFields][kStage];
     __shared__ uint16_t s_cw[kStage];
     __shared__ uint32_t s_rb[RC];
     __shared__ uint32_t s_pre[RC + 1];
+    __shared__ int s_next;
 
-    uint32_t err = 0;
-    double time = 0.0;
-    bool rec = false;
-    float ramp = 1.f;
-    if constexpr (FUSED) {
-        resolve_step(S, P, time, rec);
-        ramp = current_ramp(time);
-        // node velocities of T +- 2: every node's loads first (coordinates
-        // clamped into the grid, so all loads are unconditional), then the update
-        constexpr int NIT = (Fr::N + kTileThreads - 1) / kTileThreads;
-        float nm[NIT], nmx[NIT], nmy[NIT], nwx[NIT], nwy[NIT], ncx[NIT], ncy[NIT];
-        uint8_t nst[NIT];
-#pragma unroll
-        for (int it = 0; it < NIT; it++) {
-            const int k = min(it * kTileThreads + (int)threadIdx.x, Fr::N - 1);
-            const int lx = k / Fr::H, ly = k - (k / Fr::H) * Fr::H;
-            const int x = min(max(fx0 + lx, 0), P.gx - 1), y = min(max(fy0 + ly, 0), P.gy - 1);
-            frame_node_sum<TW, TH>(Fc, dd, x, y, nm[it], nmx[it], nmy[it]);
-            const size_t node = (size_t)x * P.gy + y;
-            nst[it] = G.status[node];
-            nwx[it] = G.wind_x[node];
-            nwy[it] = G.wind_y[node];
-            ncx[it] = P.has_current ? G.cur_x[node] : 0.f;
-            ncy[it] = P.has_current ? G.cur_y[node] : 0.f;
+    // Workgroup b takes tile-table entry xcd_remap(b).  PERSIST: the launch is
+    // persistent, workgroup b (on XCD b % 8) takes the next entry of that
+    // XCD's contiguous run of the table from queue[b % 8] (zeroed before the
+    // launch) until the run is exhausted; the grid is a multiple of 8.
+    for (int iter = 0;; iter++) {
+        int entry;
+        if constexpr (PERSIST) {
+            const int xcd = (int)(blockIdx.x % 8u), per = ntiles / 8, rem = ntiles % 8;
+            __syncthreads();   // the previous tile is done with every shared array
+            if (threadIdx.x == 0) s_next = (int)atomicAdd(&queue[xcd], 1u);
+            __syncthreads();
+            const int j = s_next;   // uniform
+            if (j >= per + (xcd < rem ? 1 : 0)) break;
+            entry = xcd * per + min(xcd, rem) + j;
+        } else {
+            if (iter > 0) break;
+            entry = xcd_remap(blockIdx.x, ntiles);
         }
+        const TileRef tr = T[entry];
+        const DomainDesc dd = D[tr.d];
+        DevParams P = P0;
+        P.gx = dd.gx;
+        P.gy = dd.gy;
+        P.has_current = dd.has_current;
+        const Lines L{L0.base + dd.pt_base, L0.pitch};
+        const uint32_t *__restrict__ cs = cs0 + dd.cs_base;
+        const size_t nb = dd.node_base;
+        const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
+                          G0.wind_x + nb, G0.wind_y + nb, G0.status + nb};
+        Counters *cnt = cnt0 + tr.d;
+        uint32_t *__restrict__ offtile = off0 + dd.off_base;
+        const int x0 = tr.tx * TW, y0 = tr.ty * TH;
+        const int fx0 = x0 - 2, fy0 = y0 - 2;                // frame origin
+
+        uint32_t err = 0;
+        double time = 0.0;
+        bool rec = false;
+        float ramp = 1.f;
+        if constexpr (FUSED) {
+            resolve_step(S, P, time, rec);
+            ramp = current_ramp(time);
+            // node velocities of T +- 2: every node's loads first (coordinates
+            // clamped into the grid, so all loads are unconditional), then the update
+            constexpr int NIT = (Fr::N + kTileThreads - 1) / kTileThreads;
+            float nm[NIT], nmx[NIT], nmy[NIT], nwx[NIT], nwy[NIT], ncx[NIT], ncy[NIT];
+            uint8_t nst[NIT];
 #pragma unroll
-        for (int it = 0; it < NIT; it++) {
-            const int k = it * kTileThreads + (int)threadIdx.x;
-            if (k < Fr::N) {
+            for (int it = 0; it < NIT; it++) {
+                const int k = min(it * kTileThreads + (int)threadIdx.x, Fr::N - 1);
                 const int lx = k / Fr::H, ly = k - (k / Fr::H) * Fr::H;
-                const int x = fx0 + lx, y = fy0 + ly;
-                float2 v = make_float2(0.f, 0.f);
-                if (x >= 0 && x < P.gx && y >= 0 && y < P.gy) {
-                    bool wall;
-                    node_velocity(nm[it], nmx[it], nmy[it], nst[it], ncx[it], ncy[it], nwx[it], nwy[it], ramp, P, v.x,
-                                  v.y, wall);
-                    if (lx >= 2 && lx < TW + 2 && ly >= 2 && ly < TH + 2) {   // T's own node
-                        const size_t node = (size_t)x * P.gy + y;
-                        if (wall) { G.fx[node] += nmx[it]; G.fy[node] += nmy[it]; }
-                        if (nm[it] != 0.f && (__builtin_isnan(v.x) || __builtin_isnan(v.y))) err |= ICE_ERR_GRID_NAN;
+                const int x = min(max(fx0 + lx, 0), P.gx - 1), y = min(max(fy0 + ly, 0), P.gy - 1);
+                frame_node_sum<TW, TH>(Fc, dd, x, y, nm[it], nmx[it], nmy[it]);
+                const size_t node = (size_t)x * P.gy + y;
+                nst[it] = G.status[node];
+                nwx[it] = G.wind_x[node];
+                nwy[it] = G.wind_y[node];
+                ncx[it] = P.has_current ? G.cur_x[node] : 0.f;
+                ncy[it] = P.has_current ? G.cur_y[node] : 0.f;
+            }
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int k = it * kTileThreads + (int)threadIdx.x;
+                if (k < Fr::N) {
+                    const int lx = k / Fr::H, ly = k - (k / Fr::H) * Fr::H;
+                    const int x = fx0 + lx, y = fy0 + ly;
+                    float2 v = make_float2(0.f, 0.f);
+                    if (x >= 0 && x < P.gx && y >= 0 && y < P.gy) {
+                        bool wall;
+                        node_velocity(nm[it], nmx[it], nmy[it], nst[it], ncx[it], ncy[it], nwx[it], nwy[it], ramp, P, v.x,
+                                      v.y, wall);
+                        if (lx >= 2 && lx < TW + 2 && ly >= 2 && ly < TH + 2) {   // T's own node
+                            const size_t node = (size_t)x * P.gy + y;
+                            if (wall) { G.fx[node] += nmx[it]; G.fy[node] += nmy[it]; }
+                            if (nm[it] != 0.f && (__builtin_isnan(v.x) || __builtin_isnan(v.y))) err |= ICE_ERR_GRID_NAN;
+                        }
                     }
+                    s_v[k] = v;
                 }
-                s_v[k] = v;
             }
         }
-    }
-    for (int k = threadIdx.x; k < 3 * Fr::NN; k += kTileThreads) s_acc[k] = 0.f;
+        for (int k = threadIdx.x; k < 3 * Fr::NN; k += kTileThreads) s_acc[k] = 0.f;
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int rd = 0; rd < NROUND; rd++) {
-        const int col0 = rd * RC;
-        const int ncol = min(RC, TW - col0);
-        column_ranges(cs, P, x0 + col0, ncol, y0, y0 + TH, s_rb, s_pre);   // contains __syncthreads
-        const uint32_t n_round = s_pre[ncol];
-        // this lane's cell and the flattened range of its points
-        const int cc = (int)threadIdx.x / TH, rr = (int)threadIdx.x - cc * TH;
-        const int cx = x0 + col0 + cc, cy = y0 + rr;
-        uint32_t f0 = 0, flen = 0;
-        if (cc < ncol && cx < P.gx && cy < P.gy) {
-            const size_t cell = (size_t)cx * P.gy + cy;
-            const uint32_t b = cs[cell];
-            f0 = s_pre[cc] + (b - s_rb[cc]);
-            flen = cs[cell + 1] - b;
-        }
-        for (uint32_t sub = 0; sub < n_round; sub += kStage) {   // uniform bound
-            // locate this lane's point of each pass (slot, or ~0u for none)
-            uint32_t slot[kPasses];
-#pragma unroll
-            for (int p = 0; p < kPasses; p++) {
-                slot[p] = ~0u;
-                const uint32_t jw = sub + (uint32_t)(p * kTileThreads + wave * 64);
-                if (jw < n_round) {   // wave-uniform
-                    const int cb = flat_col_n(jw, s_pre, ncol);
-                    const uint32_t j = jw + lane;
-                    if (j < n_round) {
-                        const int c = col_from_n(j, cb, s_pre, ncol);
-                        slot[p] = s_rb[c] + (j - s_pre[c]);
-                    }
-                }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int rd = 0; rd < NROUND; rd++) {
+            const int col0 = rd * RC;
+            const int ncol = min(RC, TW - col0);
+            column_ranges(cs, P, x0 + col0, ncol, y0, y0 + TH, s_rb, s_pre);   // contains __syncthreads
+            const uint32_t n_round = s_pre[ncol];
+            // this lane's cell and the flattened range of its points
+            const int cc = (int)threadIdx.x / TH, rr = (int)threadIdx.x - cc * TH;
+            const int cx = x0 + col0 + cc, cy = y0 + rr;
+            uint32_t f0 = 0, flen = 0;
+            if (cc < ncol && cx < P.gx && cy < P.gy) {
+                const size_t cell = (size_t)cx * P.gy + cy;
+                const uint32_t b = cs[cell];
+                f0 = s_pre[cc] + (b - s_rb[cc]);
+                flen = cs[cell + 1] - b;
             }
-            if constexpr (FUSED) {
-                // every pass's loads first, then the computation (the surrogate
-                // variant, short of registers, loads each pass when it computes it)
-                G2PRaw raw[kPasses];
-                if constexpr (!SURR) {
-#pragma unroll
-                    for (int p = 0; p < kPasses; p++)
-                        if (slot[p] != ~0u) load_g2p_raw(L, P, slot[p], raw[p]);
-                }
+            for (uint32_t sub = 0; sub < n_round; sub += kStage) {   // uniform bound
+                // locate this lane's point of each pass (slot, or ~0u for none)
+                uint32_t slot[kPasses];
 #pragma unroll
                 for (int p = 0; p < kPasses; p++) {
-                    const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;   // staging slot
-                    bool stage = false;
-                    P2GPoint q;
-                    const uint32_t i = slot[p];
-                    G2PRaw rw;
-                    if constexpr (SURR) {
-                        if (i != ~0u) load_g2p_raw(L, P, i, rw);
-                    } else {
-                        rw = raw[p];
+                    slot[p] = ~0u;
+                    const uint32_t jw = sub + (uint32_t)(p * kTileThreads + wave * 64);
+                    if (jw < n_round) {   // wave-uniform
+                        const int cb = flat_col_n(jw, s_pre, ncol);
+                        const uint32_t j = jw + lane;
+                        if (j < n_round) {
+                            const int c = col_from_n(j, cb, s_pre, ncol);
+                            slot[p] = s_rb[c] + (j - s_pre[c]);
+                        }
                     }
-                    bool act = i != ~0u && !(rw.u & (kDisabled | kEscaped));
-                    G2PState s;
-                    s.u = rw.u;
-                    s.cx = (int)(rw.cell & 0xffff);
-                    s.cy = (int)(rw.cell >> 16);
-                    s.x = rw.x;
-                    s.y = rw.y;
-                    s.Fe = M2{rw.f00, rw.f01, rw.f10, rw.f11};
-                    s.Jp = rw.jp;
-                    const float th = rw.th;
-                    float pvx = 0.f, pvy = 0.f;
-                    M2 C{0.f, 0.f, 0.f, 0.f};
-                    FInv vf;
-                    if (act) {
-                        const int rx = s.cx - fx0, ry = s.cy - fy0;
-                        if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2) {
-                            err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
-                            act = false;
+                }
+                if constexpr (FUSED) {
+                    // every pass's loads first, then the computation (the surrogate
+                    // variant, short of registers, loads each pass when it computes it)
+                    G2PRaw raw[kPasses];
+                    if constexpr (!SURR) {
+#pragma unroll
+                        for (int p = 0; p < kPasses; p++)
+                            if (slot[p] != ~0u) load_g2p_raw(L, P, slot[p], raw[p]);
+                    }
+#pragma unroll
+                    for (int p = 0; p < kPasses; p++) {
+                        const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;   // staging slot
+                        bool stage = false;
+                        P2GPoint q;
+                        const uint32_t i = slot[p];
+                        G2PRaw rw;
+                        if constexpr (SURR) {
+                            if (i != ~0u) load_g2p_raw(L, P, i, rw);
                         } else {
-                            float wx[3], wy[3];
-                            weights(s.x, wx);
-                            weights(s.y, wy);
-#pragma unroll
-                            for (int a = -1; a <= 1; a++)
-#pragma unroll
-                                for (int b = -1; b <= 1; b++) {
-                                    const float w = wx[a + 1] * wy[b + 1];
-                                    const float2 nv = s_v[(rx + a) * Fr::H + (ry + b)];
-                                    const float dx = (float)a - s.x, dy = (float)b - s.y;
-                                    pvx += w * nv.x;
-                                    pvy += w * nv.y;
-                                    const float f = P.four_h_inv * w;
-                                    C.a00 += f * (nv.x * dx);
-                                    C.a01 += f * (nv.x * dy);
-                                    C.a10 += f * (nv.y * dx);
-                                    C.a11 += f * (nv.y * dy);
-                                }
-                            g2p_predict(P, s, pvx, pvy, C, err, vf);
+                            rw = raw[p];
                         }
-                    }
-                    bool exact_rm = act && (s.u & kCrushed);
-                    if constexpr (SURR) {
-                        // the plastic branch through the learned projection, one
-                        // wave of points per MFMA batch (all lanes take part)
-                        // the plastic branch with the ridge cap active (what it was trained on)
-                        bool want = exact_rm && th == 1.f && s.p_tr >= rw.dpt &&
-                                    s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, rw.dpt, P);
-                        float xr[kMlpIn] = {0.f, 0.f, 1.f, 0.f};
-                        if (want) {
-                            mlp_features(vf, s.Jp, -rw.dpt * P.tan_phi, xr);
-                            want = xr[0] >= SA.lj0 && xr[0] <= SA.lj1 && xr[1] >= SA.d0 && xr[1] <= SA.d1 &&
-                                   xr[2] >= SA.jp0 && xr[2] <= SA.jp1;
-                        }
-                        if (__any(want)) {
-                            float xs[kMlpIn];
-#pragma unroll
-                            for (int k = 0; k < kMlpIn; k++) xs[k] = want ? (xr[k] - SA.N.xm[k]) / SA.N.xs[k] : 0.f;
-                            float y[kMlpOut];
-                            mlp_infer(SA.M, xs, y);
-                            bool accepted = false;
-                            if (want) {
-                                float s0, s1, jpf, pq, qq, viol;
-                                accepted = mlp_decode(SA.N, y, s.Jp, rw.dpt, th, P, SA.tol, s0, s1, jpf, pq, qq, viol);
-                                if (accepted) {
-                                    s.Fe = rebuild(vf, s0, s1);
-                                    s.Jp *= jpf;
-                                    vf = invariants(s.Fe);
-                                    if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
-                                    exact_rm = false;
-                                }
-                            }
-                            const uint64_t ba = __ballot(accepted), bw = __ballot(want);
-                            if (lane == 0) {
-                                atomicAdd(&cnt->pad[0], (uint32_t)__popcll(ba));
-                                atomicAdd(&cnt->pad[1], (uint32_t)__popcll(bw & ~ba));
-                            }
-                        }
-                    }
-                    if (act) {
-                        if (exact_rm) g2p_return_map(P, s, th, rw.dpt, err, vf);
-                        if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
-                        bool escaped = false;
-                        if (s.cell_updated && !(s.u & kDisabled)) {
-                            if (s.u & kMoved) {
-                                s.u |= kEscaped | kFresh;
-                                const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
-                                offtile[e] = i;
-                                escaped = true;
+                        bool act = i != ~0u && !(rw.u & (kDisabled | kEscaped));
+                        G2PState s;
+                        s.u = rw.u;
+                        s.cx = (int)(rw.cell & 0xffff);
+                        s.cy = (int)(rw.cell >> 16);
+                        s.x = rw.x;
+                        s.y = rw.y;
+                        s.Fe = M2{rw.f00, rw.f01, rw.f10, rw.f11};
+                        s.Jp = rw.jp;
+                        const float th = rw.th;
+                        float pvx = 0.f, pvy = 0.f;
+                        M2 C{0.f, 0.f, 0.f, 0.f};
+                        FInv vf;
+                        if (act) {
+                            const int rx = s.cx - fx0, ry = s.cy - fy0;
+                            if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2) {
+                                err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
+                                act = false;
                             } else {
-                                s.u |= kMoved;
+                                float wx[3], wy[3];
+                                weights(s.x, wx);
+                                weights(s.y, wy);
+#pragma unroll
+                                for (int a = -1; a <= 1; a++)
+#pragma unroll
+                                    for (int b = -1; b <= 1; b++) {
+                                        const float w = wx[a + 1] * wy[b + 1];
+                                        const float2 nv = s_v[(rx + a) * Fr::H + (ry + b)];
+                                        const float dx = (float)a - s.x, dy = (float)b - s.y;
+                                        pvx += w * nv.x;
+                                        pvy += w * nv.y;
+                                        const float f = P.four_h_inv * w;
+                                        C.a00 += f * (nv.x * dx);
+                                        C.a01 += f * (nv.x * dy);
+                                        C.a10 += f * (nv.y * dx);
+                                        C.a11 += f * (nv.y * dy);
+                                    }
+                                g2p_predict(P, s, pvx, pvy, C, err, vf);
                             }
                         }
-                        // escaped and disabled points leave the pipeline: store all
-                        g2p_store_pipelined(L, i, s, rw.u, rw.jp, pvx, pvy, C, rec,
-                                            S.complete || escaped || (s.u & kDisabled));
-                        if (!(s.u & kDisabled) && !escaped) {
-                            p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);
-                            stage = true;
+                        bool exact_rm = act && (s.u & kCrushed);
+                        if constexpr (SURR) {
+                            // the plastic branch through the learned projection, one
+                            // wave of points per MFMA batch (all lanes take part)
+                            // the plastic branch with the ridge cap active (what it was trained on)
+                            bool want = exact_rm && th == 1.f && s.p_tr >= rw.dpt &&
+                                        s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, rw.dpt, P);
+                            float xr[kMlpIn] = {0.f, 0.f, 1.f, 0.f};
+                            if (want) {
+                                mlp_features(vf, s.Jp, -rw.dpt * P.tan_phi, xr);
+                                want = xr[0] >= SA.lj0 && xr[0] <= SA.lj1 && xr[1] >= SA.d0 && xr[1] <= SA.d1 &&
+                                       xr[2] >= SA.jp0 && xr[2] <= SA.jp1;
+                            }
+                            if (__any(want)) {
+                                float xs[kMlpIn];
+#pragma unroll
+                                for (int k = 0; k < kMlpIn; k++) xs[k] = want ? (xr[k] - SA.N.xm[k]) / SA.N.xs[k] : 0.f;
+                                float y[kMlpOut];
+                                mlp_infer(SA.M, xs, y);
+                                bool accepted = false;
+                                if (want) {
+                                    float s0, s1, jpf, pq, qq, viol;
+                                    accepted = mlp_decode(SA.N, y, s.Jp, rw.dpt, th, P, SA.tol, s0, s1, jpf, pq, qq, viol);
+                                    if (accepted) {
+                                        s.Fe = rebuild(vf, s0, s1);
+                                        s.Jp *= jpf;
+                                        vf = invariants(s.Fe);
+                                        if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
+                                        exact_rm = false;
+                                    }
+                                }
+                                const uint64_t ba = __ballot(accepted), bw = __ballot(want);
+                                if (lane == 0) {
+                                    atomicAdd(&cnt->pad[0], (uint32_t)__popcll(ba));
+                                    atomicAdd(&cnt->pad[1], (uint32_t)__popcll(bw & ~ba));
+                                }
+                            }
                         }
-                    }
-                    stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
-                }
-            } else {
-#pragma unroll
-                for (int p = 0; p < kPasses; p++) {
-                    const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;
-                    bool stage = false;
-                    P2GPoint q;
-                    const uint32_t i = slot[p];
-                    if (i != ~0u) {
-                        const uint32_t u = L.u(ICE_IDX_UTILITY, i);
-                        if (!(u & (kDisabled | kEscaped))) {
-                            load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
-                            if (!stencil_inside(q.cx, q.cy, P)) err |= ICE_ERR_POINT_LEFT_AREA;
-                            else stage = true;
+                        if (act) {
+                            if (exact_rm) g2p_return_map(P, s, th, rw.dpt, err, vf);
+                            if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
+                            bool escaped = false;
+                            if (s.cell_updated && !(s.u & kDisabled)) {
+                                if (s.u & kMoved) {
+                                    s.u |= kEscaped | kFresh;
+                                    const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
+                                    offtile[e] = i;
+                                    escaped = true;
+                                } else {
+                                    s.u |= kMoved;
+                                }
+                            }
+                            // escaped and disabled points leave the pipeline: store all
+                            g2p_store_pipelined(L, i, s, rw.u, rw.jp, pvx, pvy, C, rec,
+                                                S.complete || escaped || (s.u & kDisabled));
+                            if (!(s.u & kDisabled) && !escaped) {
+                                p2g_terms(P, s.x, s.y, s.cx, s.cy, pvx, pvy, vf, C, s.Jp, th, q);
+                                stage = true;
+                            }
                         }
+                        stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
                     }
-                    stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
+                } else {
+#pragma unroll
+                    for (int p = 0; p < kPasses; p++) {
+                        const uint32_t jj = (uint32_t)(p * kTileThreads) + threadIdx.x;
+                        bool stage = false;
+                        P2GPoint q;
+                        const uint32_t i = slot[p];
+                        if (i != ~0u) {
+                            const uint32_t u = L.u(ICE_IDX_UTILITY, i);
+                            if (!(u & (kDisabled | kEscaped))) {
+                                load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
+                                if (!stencil_inside(q.cx, q.cy, P)) err |= ICE_ERR_POINT_LEFT_AREA;
+                                else stage = true;
+                            }
+                        }
+                        stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
+                    }
                 }
-            }
-            __syncthreads();
-            // each cell lane sums the staged points sorted into its cell
-            float am[3][3], ax[3][3], ay[3][3];
+                __syncthreads();
+                // each cell lane sums the staged points sorted into its cell
+                float am[3][3], ax[3][3], ay[3][3];
 #pragma unroll
-            for (int a = 0; a < 3; a++)
+                for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) am[a][b] = ax[a][b] = ay[a][b] = 0.f;
-            bool any = false;
-            const uint32_t lo = max(f0, sub), hi = min(f0 + flen, sub + (uint32_t)kStage);
-            for (uint32_t jf = lo; jf < hi; jf++) {
-                const uint32_t k = jf - sub;
-                P2GPoint t;
-                t.mass = s_st[2][k];
-                if (t.mass == 0.f) continue;
-                t.x = s_st[0][k];
-                t.y = s_st[1][k];
-                t.mvx = s_st[3][k];
-                t.mvy = s_st[4][k];
-                t.s00 = s_st[5][k];
-                t.s01 = s_st[6][k];
-                t.s10 = s_st[7][k];
-                t.s11 = s_st[8][k];
-                float wx[3], wy[3];
-                weights(t.x, wx);
-                weights(t.y, wy);
-                const uint32_t pc = s_cw[k];
-                const int sx = (int)(pc & 0xffu) - (cx - fx0), sy = (int)(pc >> 8) - (cy - fy0);
-                if ((sx | sy) != 0 && !moved_stencil<TW, TH>(t, sx, sy, cx - fx0, cy - fy0, wx, wy, s_acc, err))
-                    continue;
+                    for (int b = 0; b < 3; b++) am[a][b] = ax[a][b] = ay[a][b] = 0.f;
+                bool any = false;
+                const uint32_t lo = max(f0, sub), hi = min(f0 + flen, sub + (uint32_t)kStage);
+                for (uint32_t jf = lo; jf < hi; jf++) {
+                    const uint32_t k = jf - sub;
+                    P2GPoint t;
+                    t.mass = s_st[2][k];
+                    if (t.mass == 0.f) continue;
+                    t.x = s_st[0][k];
+                    t.y = s_st[1][k];
+                    t.mvx = s_st[3][k];
+                    t.mvy = s_st[4][k];
+                    t.s00 = s_st[5][k];
+                    t.s01 = s_st[6][k];
+                    t.s10 = s_st[7][k];
+                    t.s11 = s_st[8][k];
+                    float wx[3], wy[3];
+                    weights(t.x, wx);
+                    weights(t.y, wy);
+                    const uint32_t pc = s_cw[k];
+                    const int sx = (int)(pc & 0xffu) - (cx - fx0), sy = (int)(pc >> 8) - (cy - fy0);
+                    if ((sx | sy) != 0 && !moved_stencil<TW, TH>(t, sx, sy, cx - fx0, cy - fy0, wx, wy, s_acc, err))
+                        continue;
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) {
+                            float dm, dpx, dpy;
+                            p2g_contrib(t, a - 1, b - 1, wx[a] * wy[b], dm, dpx, dpy);
+                            am[a][b] += dm;
+                            ax[a][b] += dpx;
+                            ay[a][b] += dpy;
+                        }
+                    any = true;
+                }
+                if (any) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) {
+                            if (__builtin_isnan(ax[a][b]) || __builtin_isnan(ay[a][b])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+                            if (__builtin_isnan(am[a][b])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+                        }
+                }
+                __syncthreads();   // staged terms consumed: the area now takes the cell stencils
+                float *s_cs = &s_st[0][0];   // [27][kTileThreads]: component-major, offset (a, b), cell lane
 #pragma unroll
                 for (int a = 0; a < 3; a++)
 #pragma unroll
                     for (int b = 0; b < 3; b++) {
-                        float dm, dpx, dpy;
-                        p2g_contrib(t, a - 1, b - 1, wx[a] * wy[b], dm, dpx, dpy);
-                        am[a][b] += dm;
-                        ax[a][b] += dpx;
-                        ay[a][b] += dpy;
+                        const int q = a * 3 + b;
+                        s_cs[q * kTileThreads + threadIdx.x] = am[a][b];
+                        s_cs[(9 + q) * kTileThreads + threadIdx.x] = ax[a][b];
+                        s_cs[(18 + q) * kTileThreads + threadIdx.x] = ay[a][b];
                     }
-                any = true;
-            }
-            if (any) {
+                __syncthreads();
+                // each node the round touches gathers its (up to) nine cell stencils
+                // in a fixed order: frame columns col0 + 1 .. col0 + ncol + 2, rows 1 .. TH + 2
+                const int nyn = TH + 2;
+                for (int k = threadIdx.x; k < (ncol + 2) * nyn; k += kTileThreads) {
+                    const int ix = k / nyn, iy = k - ix * nyn;
+                    float sm = 0.f, sx = 0.f, sy = 0.f;
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                    for (int a = 0; a < 3; a++) {
+                        const int ccn = ix - a;
+                        if (ccn < 0 || ccn >= ncol) continue;
 #pragma unroll
-                    for (int b = 0; b < 3; b++) {
-                        if (__builtin_isnan(ax[a][b]) || __builtin_isnan(ay[a][b])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-                        if (__builtin_isnan(am[a][b])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+                        for (int b = 0; b < 3; b++) {
+                            const int rrn = iy - b;
+                            if (rrn < 0 || rrn >= TH) continue;
+                            const int c = ccn * TH + rrn, q = a * 3 + b;
+                            sm += s_cs[q * kTileThreads + c];
+                            sx += s_cs[(9 + q) * kTileThreads + c];
+                            sy += s_cs[(18 + q) * kTileThreads + c];
+                        }
                     }
-            }
-            __syncthreads();   // staged terms consumed: the area now takes the cell stencils
-            float *s_cs = &s_st[0][0];   // [27][kTileThreads]: component-major, offset (a, b), cell lane
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    const int q = a * 3 + b;
-                    s_cs[q * kTileThreads + threadIdx.x] = am[a][b];
-                    s_cs[(9 + q) * kTileThreads + threadIdx.x] = ax[a][b];
-                    s_cs[(18 + q) * kTileThreads + threadIdx.x] = ay[a][b];
+                    const int o = (col0 + 1 + ix) * Fr::LP + (1 + iy);
+                    s_acc[o] += sm;
+                    s_acc[Fr::NN + o] += sx;
+                    s_acc[2 * Fr::NN + o] += sy;
                 }
-            __syncthreads();
-            // each node the round touches gathers its (up to) nine cell stencils
-            // in a fixed order: frame columns col0 + 1 .. col0 + ncol + 2, rows 1 .. TH + 2
-            const int nyn = TH + 2;
-            for (int k = threadIdx.x; k < (ncol + 2) * nyn; k += kTileThreads) {
-                const int ix = k / nyn, iy = k - ix * nyn;
-                float sm = 0.f, sx = 0.f, sy = 0.f;
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    const int ccn = ix - a;
-                    if (ccn < 0 || ccn >= ncol) continue;
-#pragma unroll
-                    for (int b = 0; b < 3; b++) {
-                        const int rrn = iy - b;
-                        if (rrn < 0 || rrn >= TH) continue;
-                        const int c = ccn * TH + rrn, q = a * 3 + b;
-                        sm += s_cs[q * kTileThreads + c];
-                        sx += s_cs[(9 + q) * kTileThreads + c];
-                        sy += s_cs[(18 + q) * kTileThreads + c];
-                    }
-                }
-                const int o = (col0 + 1 + ix) * Fr::LP + (1 + iy);
-                s_acc[o] += sm;
-                s_acc[Fr::NN + o] += sx;
-                s_acc[2 * Fr::NN + o] += sy;
+                __syncthreads();   // stencils read: the area takes the next staged terms
             }
-            __syncthreads();   // stencils read: the area takes the next staged terms
+            __syncthreads();   // the round's ranges are read; the next round may overwrite them
         }
-        __syncthreads();   // the round's ranges are read; the next round may overwrite them
+        // the frame, every node once (coalesced)
+        float *__restrict__ out = Fn + frame_offset<TW, TH>(dd, tr.tx, tr.ty);
+        for (int k = threadIdx.x; k < 3 * Fr::N; k += kTileThreads) {
+            const int comp = k / Fr::N, nd = k - comp * Fr::N;
+            const int lx = nd / Fr::H, ly = nd - lx * Fr::H;
+            out[k] = s_acc[comp * Fr::NN + lx * Fr::LP + ly];
+        }
+        flush_err(cnt, err);
     }
-    // the frame, every node once (coalesced)
-    float *__restrict__ out = Fn + frame_offset<TW, TH>(dd, tr.tx, tr.ty);
-    for (int k = threadIdx.x; k < 3 * Fr::N; k += kTileThreads) {
-        const int comp = k / Fr::N, nd = k - comp * Fr::N;
-        const int lx = nd / Fr::H, ly = nd - lx * Fr::H;
-        out[k] = s_acc[comp * Fr::NN + lx * Fr::LP + ly];
-    }
-    flush_err(cnt, err);
 }
 
 // The off-tile points after a pipeline pass.  do_g2p = 1 (fused substep):

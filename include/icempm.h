@@ -215,7 +215,9 @@ int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
 /* Tuning switches of the sorted path (no reference counterpart): "band"
  * (1..4096, default 4) is the number of tile columns per band of the
  * workgroup-to-tile order; "pipelined" (0 or 1, default 1) selects the
- * pipelined substep of ice_substep / ice_run. */
+ * pipelined substep of ice_substep / ice_run; "persistent" (0 or 1, default
+ * 0) runs its tile passes (without the surrogate) as persistent launches, as
+ * many workgroups as fit on the device pulling tiles from per-XCD queues. */
 int ice_set_option(IceHandle *h, const char *name, int value);
 /* Tile autotuner (no reference counterpart): times `nsteps` substeps for each
  * tile shape from 8x8 to 64x32, restoring the simulation state after each

@@ -62,6 +62,26 @@ def test_pipelined_graph_matches_eager():
     b.close()
 
 
+@pytest.mark.parametrize("tile", [(16, 32), (8, 8)])
+def test_persistent_launch_matches_one_workgroup_per_tile(tile):
+    """Option "persistent": a fixed grid of workgroups pulling tiles from
+    per-XCD queues gives the result of one workgroup per tile (48 substeps,
+    graph replays, re-sorts), within the GPU-vs-GPU bar."""
+    _print("test_persistent_launch_matches_one_workgroup_per_tile", tile)
+    sc = scenes.dynamic_scene(seed=8)
+    a = Simulation(sc, tile=tile)
+    b = Simulation(sc, tile=tile, options={"persistent": 1})
+    a.run(0, 48, graph=True)
+    b.run(0, 48, graph=True)
+    pa, fa, ea, da = a.download()
+    pb, fb, eb, db = b.download()
+    _print(compare_points(pa, pb, fields=GPU_VS_GPU, max_bad_frac=1e-3, max_flag_mismatch=1e-3))
+    assert ea == eb == 0 and da == db
+    np.testing.assert_allclose(fa, fb, rtol=1e-4, atol=1e-6 * max(np.abs(fb).max(), 1.0))
+    a.close()
+    b.close()
+
+
 def test_pipelined_run_with_resorts_matches_oracle():
     """ice_run over 96 substeps of a fast scene: the off-tile list outgrows
     its limit, so the run re-sorts between graph chunks; the frames carry the
