@@ -61,6 +61,14 @@ def test_snapshot_round_trip_bits(tmp_path, precision):
     assert np.array_equal(back19.view(np.uint32), soa[:19].view(np.uint32))
 
 
+def test_empty_snapshot_round_trip(tmp_path):
+    path = str(tmp_path / "e.h5")
+    h5io.snapshot_save(path, np.zeros((20, 0), np.float32), 3, 0.5, 32)
+    assert h5io.snapshot_info(path)["n"] == 0
+    soa, step, t = h5io.snapshot_load(path, abi.N_LINES_MAX)
+    assert soa.shape == (20, 0) and (step, t) == (3, 0.5)
+
+
 def test_snapshot_layout_matches_the_reference(tmp_path):
     soa = random_soa(300000, seed=1)
     path = str(tmp_path / "s.h5")
