@@ -1,7 +1,8 @@
 """ctypes binding of the oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+cpu_baseline leg of bench.py, to check results against.  The product never
+imports it.
 """
 from __future__ import annotations
 

@@ -7,7 +7,7 @@
 // It restates igorg520b/iceMPM_2d_sp's substep math, Eigen-free, templated on
 // the arithmetic type T (double = the reference precision, float = a plain
 // FP32 restatement used to show the FP32 floor).  Each function cites the
-// reference file:line it follows.  Two additions of this build are restated
+// reference file:line it follows.  Two additions beyond the reference are restated
 // here too, since the reference does not have them: the wind-stress term in
 // the node update and the optional per-particle cohesion (DP threshold).
 //
@@ -392,7 +392,7 @@ template <class T> struct State {
     }
 
     // partition_kernel_update_nodes (kernels.cuh:109-177) plus the wind-stress
-    // term of this build: v += dt * windStressToAccel * tau_wind on water nodes,
+    // term (not in the reference): v += dt * windStressToAccel * tau_wind on water nodes,
     // after the water drag and before the velocity clamp.
     void update_nodes(double simulation_time) {
         const T dt = T(prm.InitialTimeStep);

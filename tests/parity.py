@@ -3,7 +3,7 @@
 Points are matched by their ICE_IDX_POINT line (the device may reorder them by
 cell).  Float fields are compared as |a - b| <= atol + rtol * scale, where
 scale is the field's largest magnitude in the reference (so nearly-zero entries
-are judged against the field's size, not their own).
+are measured against the field's size, not their own).
 """
 from __future__ import annotations
 
