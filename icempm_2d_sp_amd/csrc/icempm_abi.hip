@@ -423,9 +423,11 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
     }
     hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(NC, 256)), dim3(256), 0, h->stream, NC, cs, h->s_order);
     LAUNCHCHK(h, "k_sort_cells");
-    uint32_t n_active = 0;
-    HIPCHK(h, hipMemcpyAsync(&n_active, cs + NC, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    uint32_t n_active = n;   // only a sort that drops the disabled points needs the count on the host
+    if (drop) {
+        HIPCHK(h, hipMemcpyAsync(&n_active, cs + NC, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
     const uint32_t n_out = drop ? n_active : n;
     if (n_out) {
         Lines dst{h->pts_alt + D.pt_base, h->pitch};
@@ -443,13 +445,9 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
     drop_graph(h);   // a captured graph holds the old point buffer
     D.n = n_out;
     Counters *c = h->dcnt(d);
-    if (drop) {
-        const int32_t zero = 0;
-        HIPCHK(h, hipMemcpyAsync(&c->disabled, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-    }
-    const uint32_t zero_u = 0;
-    HIPCHK(h, hipMemcpyAsync(&c->n_esc, &zero_u, sizeof(zero_u), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (drop && (rc = launch_fill_u32(h, reinterpret_cast<uint32_t *>(&c->disabled), 1, 0u))) return rc;
+    if ((rc = launch_fill_u32(h, &c->n_esc, 1, 0u))) return rc;
+    if (drop) HIPCHK(h, hipStreamSynchronize(h->stream));   // ice_sort's caller reads the result at once
     D.sorted = true;
     D.steps_since_sort = 0;
     D.n_sorts++;
@@ -1407,7 +1405,8 @@ int ice_sort(IceHandle *h, uint32_t *n_out, uint32_t *perm) {
     if (perm && n_before) HIPCHK(h, hipMalloc(&perm_dev, sizeof(uint32_t) * n_before));
     int rc = device_sort(h, 0, true, perm_dev);
     if (rc == ICE_OK && perm && h->dom[0].n) {
-        hipError_t e = hipMemcpy(perm, perm_dev, sizeof(uint32_t) * h->dom[0].n, hipMemcpyDeviceToHost);
+        hipError_t e = hipMemcpyAsync(perm, perm_dev, sizeof(uint32_t) * h->dom[0].n, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) { h->err = hipGetErrorString(e); rc = ICE_E_HIP; }
     }
     if (perm_dev) (void)hipFree(perm_dev);
