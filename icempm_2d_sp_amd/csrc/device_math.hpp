@@ -26,10 +26,27 @@ constexpr uint32_t kCrushed = 0x10000u;
 constexpr uint32_t kDisabled = 0x20000u;
 // Bits above the reference's (which uses 0..18) that only this library uses;
 // they are cleared before points leave the device.
-constexpr uint32_t kMoved = 0x80000u;     // changed cell once since the last sort
-constexpr uint32_t kEscaped = 0x100000u;  // changed cell twice: handled off-tile
+constexpr uint32_t kMoved = 0x80000u;     // in a cell other than the one it was sorted into
+constexpr uint32_t kEscaped = 0x100000u;  // left the 3 x 3 around that cell: handled off-tile
 constexpr uint32_t kFresh = 0x200000u;    // escaped in this substep's tile pass: P2G still due
-constexpr uint32_t kInternalBits = kMoved | kEscaped | kFresh;
+// The drift from the sorted cell, dx and dy in {-1, 0, 1} as two-bit two's
+// complement fields (bits 22-23, 24-25): kMoved is set exactly when it is
+// nonzero; kFar marks a cell change that took it past one cell.
+constexpr uint32_t kDriftBits = 0x3C00000u;
+constexpr uint32_t kFar = 0x4000000u;
+constexpr uint32_t kInternalBits = kMoved | kEscaped | kFresh | kDriftBits | kFar;
+
+// Adds a cell change (dx, dy) to the drift kept in u.
+__device__ __forceinline__ void drift_cell(uint32_t &u, int dx, int dy) {
+    const int nx = (((int)(u << 8)) >> 30) + dx;   // bits 22-23, sign-extended
+    const int ny = (((int)(u << 6)) >> 30) + dy;   // bits 24-25
+    if (nx < -1 || nx > 1 || ny < -1 || ny > 1) {
+        u |= kFar;
+        return;
+    }
+    u = (u & ~(kDriftBits | kMoved)) | ((uint32_t)(nx & 3) << 22) | ((uint32_t)(ny & 3) << 24) |
+        ((nx | ny) ? kMoved : 0u);
+}
 
 // Constants the kernels read, precomputed on the host in FP64 and rounded
 // once to FP32.  Passed by value as a kernel argument.

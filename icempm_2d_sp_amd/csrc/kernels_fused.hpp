@@ -22,9 +22,10 @@
 // cell: the part of its stencil inside that lane's 3 x 3 goes into the
 // registers with shifted weights, the rest (three nodes for a move along an
 // axis, five for a diagonal one) into the LDS frame with atomics.  Points that
-// change cell a second time leave the tile pass (kEscaped + kFresh) and the
-// off-tile kernel does their P2G, with global atomics into the frame of the
-// tile that now holds them.
+// leave the 3 x 3 around their sorted cell (the drift kept in the utility
+// word) leave the tile pass (kEscaped + kFresh) and the off-tile kernel does
+// their P2G, with global atomics into the frame of the tile that now holds
+// them.
 #pragma once
 #include "tile_common.hpp"
 #include "mlp.hpp"
@@ -480,15 +481,11 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                             if (exact_rm) g2p_return_map(P, s, th, rw.dpt, err, vf);
                             if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
                             bool escaped = false;
-                            if (s.cell_updated && !(s.u & kDisabled)) {
-                                if (s.u & kMoved) {
-                                    s.u |= kEscaped | kFresh;
-                                    const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
-                                    offtile[e] = i;
-                                    escaped = true;
-                                } else {
-                                    s.u |= kMoved;
-                                }
+                            if (s.cell_updated && (s.u & (kDisabled | kFar)) == kFar) {
+                                s.u |= kEscaped | kFresh;
+                                const uint32_t e = atomicAdd(&cnt->n_esc, 1u);
+                                offtile[e] = i;
+                                escaped = true;
                             }
                             // escaped and disabled points leave the pipeline: store all
                             g2p_store_pipelined(L, i, s, rw.u, rw.jp, pvx, pvy, C, rec,

@@ -5,8 +5,9 @@
 // simulation/kernels.cuh:109-177, plus the wind-stress term) for nodes T +- 2
 // into LDS, adds wall momentum of T's own nodes to (fx, fy), then runs G2P
 // (kernels.cuh:181-315) for the points sorted into T with coalesced SoA loads
-// and stores.  A point that changes cell sets kMoved; a second change sets
-// kEscaped and appends it to the domain's off-tile list.
+// and stores.  A point whose cell changes keeps its drift from the cell it
+// was sorted into (kMoved while nonzero); one that leaves the 3 x 3 around
+// that cell sets kEscaped and is appended to the domain's off-tile list.
 #pragma once
 #include "tile_common.hpp"
 
@@ -101,14 +102,10 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiles(DevParams P0, StepAr
         FInv vf;
         g2p_finish(P, s, pvx, pvy, C, th, dpt, err, vf);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
-        if (s.cell_updated && !(s.u & kDisabled)) {
-            if (s.u & kMoved) {
-                s.u |= kEscaped;
-                const uint32_t j = atomicAdd(&cnt->n_esc, 1u);
-                offtile[j] = i;
-            } else {
-                s.u |= kMoved;
-            }
+        if (s.cell_updated && (s.u & (kDisabled | kFar)) == kFar) {
+            s.u |= kEscaped;
+            const uint32_t j = atomicAdd(&cnt->n_esc, 1u);
+            offtile[j] = i;
         }
         g2p_store(L, i, s, pvx, pvy, C, rec);
     }

@@ -119,11 +119,17 @@ __device__ __forceinline__ void g2p_predict(const DevParams &P, G2PState &s, flo
     if (__builtin_isnan(pvx) || __builtin_isnan(pvy)) err |= ICE_ERR_POINT_VEL_NAN;
     if (__builtin_isnan(s.x) || __builtin_isnan(s.y)) err |= ICE_ERR_POINT_POS_NAN;
     if (isnan4(C)) err |= ICE_ERR_POINT_BP_NAN;
-    s.cell_updated = false;
-    if (s.x > 0.5f) { s.x -= 1.f; s.cx++; s.cell_updated = true; }
-    else if (s.x < -0.5f) { s.x += 1.f; s.cx--; s.cell_updated = true; }
-    if (s.y > 0.5f) { s.y -= 1.f; s.cy++; s.cell_updated = true; }
-    else if (s.y < -0.5f) { s.y += 1.f; s.cy--; s.cell_updated = true; }
+    int dx = 0, dy = 0;
+    if (s.x > 0.5f) { s.x -= 1.f; dx = 1; }
+    else if (s.x < -0.5f) { s.x += 1.f; dx = -1; }
+    if (s.y > 0.5f) { s.y -= 1.f; dy = 1; }
+    else if (s.y < -0.5f) { s.y += 1.f; dy = -1; }
+    s.cell_updated = (dx | dy) != 0;
+    if (s.cell_updated) {
+        s.cx += dx;
+        s.cy += dy;
+        drift_cell(s.u, dx, dy);   // the sorted path's bookkeeping; no effect on the state
+    }
     s.newly_disabled = false;
     if (s.cell_updated && (s.cx <= 1 || s.cx >= P.gx - 2 || s.cy <= 1 || s.cy >= P.gy - 2)) {
         s.u |= kDisabled;
