@@ -49,6 +49,7 @@ struct Domain {
     size_t N = 0;
     uint32_t capacity = 0, n = 0;
     uint32_t pt_base = 0;          // also the base of the domain's off-tile list
+    int half = 0;                  // which half of the point buffer holds the points
     size_t node_base = 0, cs_base = 0;
     bool has_current = false;
     bool sorted = false;
@@ -66,8 +67,10 @@ struct IceHandle {
     bool batch = false;
     std::vector<Domain> dom;
 
-    // points (all domains)
-    float *pts = nullptr, *pts_alt = nullptr;
+    // points (all domains): one allocation of two halves of [ICE_N_LINES_MAX][pitch];
+    // a domain's sort gathers its points into the other half and flips Domain::half
+    float *pts = nullptr;
+    size_t half_floats = 0;
     size_t pitch = 0;
     uint32_t total_capacity = 0;
     int nlines = ICE_N_LINES;
@@ -135,7 +138,9 @@ struct IceHandle {
     int ndom() const { return (int)dom.size(); }
     float *gline(int l) const { return grid + (size_t)l * gpitch; }
     Lines lines() const { return Lines{pts, pitch}; }
-    Lines dlines(int d) const { return Lines{pts + dom[d].pt_base, pitch}; }
+    uint64_t pt_off(int d, int half) const { return (uint64_t)half * half_floats + dom[d].pt_base; }
+    float *dpts(int d) const { return pts + pt_off(d, dom[d].half); }
+    Lines dlines(int d) const { return Lines{dpts(d), pitch}; }
     float *dgline(int l, int d) const { return gline(l) + dom[d].node_base; }
     Counters *dcnt(int d) const { return cnt + d; }
     Counters *step_cnt() const { return cnt + dom.size(); }
@@ -334,8 +339,8 @@ int upload_tables(IceHandle *h) {
     for (int d = 0; d < nd; d++) {
         const Domain &D = h->dom[d];
         const int ntx = (D.gx + h->tile_w - 1) / h->tile_w, nty = (D.gy + h->tile_h - 1) / h->tile_h;
-        desc[d] = DomainDesc{D.gx,      D.gy,      D.has_current ? 1 : 0, 0, D.pt_base, D.pt_base, D.node_base,
-                             D.cs_base, ntx,       nty,                   nframes};
+        desc[d] = DomainDesc{D.gx, D.gy, D.has_current ? 1 : 0, D.pt_base, h->pt_off(d, D.half), D.node_base,
+                             D.cs_base, ntx, nty, nframes};
         nframes += (uint64_t)ntx * nty;
     }
     // frames: 3 components of (tile_w + 4) x (tile_h + 4) nodes per tile
@@ -430,16 +435,16 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
     }
     const uint32_t n_out = drop ? n_active : n;
     if (n_out) {
-        Lines dst{h->pts_alt + D.pt_base, h->pitch};
+        // into the domain's range of the other half, which then holds the points
+        const uint64_t to = h->pt_off(d, D.half ^ 1);
+        Lines dst{h->pts + to, h->pitch};
         hipLaunchKernelGGL(k_sort_gather, dim3(blocks_for(n_out, 256)), dim3(256), 0, h->stream, h->dlines(d), dst,
                            h->nlines, n_out, h->s_order, perm_dev);
         LAUNCHCHK(h, "k_sort_gather");
-        if (h->ndom() == 1) {
-            std::swap(h->pts, h->pts_alt);
-        } else {   // other domains live in the same buffers: copy the sorted range back
-            HIPCHK(h, hipMemcpy2DAsync(h->pts + D.pt_base, h->pitch * sizeof(float), h->pts_alt + D.pt_base,
-                                       h->pitch * sizeof(float), n_out * sizeof(float), h->nlines,
-                                       hipMemcpyDeviceToDevice, h->stream));
+        D.half ^= 1;
+        if (h->d_desc && !h->tables_dirty) {   // else the next table upload writes it
+            hipLaunchKernelGGL(k_set_pt_off, dim3(1), dim3(64), 0, h->stream, h->d_desc + d, to);
+            LAUNCHCHK(h, "k_set_pt_off");
         }
     }
     drop_graph(h);   // a captured graph holds the old point buffer
@@ -787,7 +792,7 @@ int alloc_points(IceHandle *h) {
     if (base >= (1ull << 30)) { h->err = "total point capacity exceeds 2^30 slots"; return ICE_E_CAPACITY; }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     drop_graph(h);
-    for (void **p : {(void **)&h->pts, (void **)&h->pts_alt, (void **)&h->offtile, (void **)&h->s_key,
+    for (void **p : {(void **)&h->pts, (void **)&h->offtile, (void **)&h->s_key,
                      (void **)&h->s_slot, (void **)&h->s_dflag, (void **)&h->s_drank, (void **)&h->s_order,
                      (void **)&h->s_bsum}) {
         if (*p) (void)hipFree(*p);
@@ -796,17 +801,18 @@ int alloc_points(IceHandle *h) {
     base = 0;
     for (auto &D : h->dom) {
         D.pt_base = (uint32_t)base;
+        D.half = 0;
         D.n = 0;
         D.sorted = false;
         base = round_up(base + std::max<uint32_t>(D.capacity, 1), 64);
     }
     h->total_capacity = (uint32_t)base;
     h->pitch = base;
-    const size_t bytes = sizeof(float) * h->pitch * ICE_N_LINES_MAX;
+    h->half_floats = h->pitch * ICE_N_LINES_MAX;
+    const size_t bytes = sizeof(float) * h->half_floats;
     const size_t mc = h->max_capacity();
-    HIPCHK(h, hipMalloc(&h->pts, bytes));
-    HIPCHK(h, hipMalloc(&h->pts_alt, bytes));
-    HIPCHK(h, hipMemsetAsync(h->pts, 0, bytes, h->stream));
+    HIPCHK(h, hipMalloc(&h->pts, 2 * bytes));
+    HIPCHK(h, hipMemsetAsync(h->pts, 0, bytes, h->stream));   // half 0, where every domain starts
     HIPCHK(h, hipMalloc(&h->offtile, sizeof(uint32_t) * h->total_capacity));
     HIPCHK(h, hipMalloc(&h->s_key, sizeof(uint32_t) * mc));
     HIPCHK(h, hipMalloc(&h->s_slot, sizeof(uint32_t) * mc));
@@ -870,7 +876,7 @@ int upload_points_d(IceHandle *h, int d, const float *soa, size_t host_pitch, ui
     HIPCHK(h, hipSetDevice(h->device));
     h->nlines = (h->prm.flags & ICE_FLAG_COHESION_LINE) ? ICE_N_LINES_MAX : ICE_N_LINES;
     if (n)
-        HIPCHK(h, hipMemcpy2DAsync(h->pts + D.pt_base, h->pitch * sizeof(float), soa, host_pitch * sizeof(float),
+        HIPCHK(h, hipMemcpy2DAsync(h->dpts(d), h->pitch * sizeof(float), soa, host_pitch * sizeof(float),
                                    n * sizeof(float), h->nlines, hipMemcpyHostToDevice, h->stream));
     D.n = n;
     // the partition's disabled count starts at 0 (gpu_implementation5.cpp:137, gpu_partition.cpp:101-103)
@@ -942,7 +948,7 @@ int download_d(IceHandle *h, int d, float *soa, size_t host_pitch, int n_lines, 
         if (n_lines < ICE_N_LINES || n_lines > ICE_N_LINES_MAX || host_pitch < D.n) { h->err = "bad layout"; return ICE_E_ARG; }
         const int nl = std::min(n_lines, h->nlines);
         if (D.n)
-            HIPCHK(h, hipMemcpy2DAsync(soa, host_pitch * sizeof(float), h->pts + D.pt_base, h->pitch * sizeof(float),
+            HIPCHK(h, hipMemcpy2DAsync(soa, host_pitch * sizeof(float), h->dpts(d), h->pitch * sizeof(float),
                                        D.n * sizeof(float), nl, hipMemcpyDeviceToHost, h->stream));
     }
     if (forces) {
@@ -1060,7 +1066,7 @@ int ice_destroy(IceHandle *h) {
     drop_graph(h);
     for (auto &t : h->ev_pool)
         for (auto &e : t.e) (void)hipEventDestroy(e);
-    for (void *p : {(void *)h->pts, (void *)h->pts_alt, (void *)h->grid, (void *)h->status, (void *)h->cnt,
+    for (void *p : {(void *)h->pts, (void *)h->grid, (void *)h->status, (void *)h->cnt,
                     (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
                     (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
@@ -1458,24 +1464,23 @@ int ice_autotune(IceHandle *h, int nsteps, int *tile_w, int *tile_h, double *bes
     int rc;
     for (int d = 0; d < h->ndom(); d++)
         if (!h->dom[d].sorted && (rc = device_sort(h, d, false, nullptr))) return rc;
-    // Save everything a trial changes (points, cell starts, force lines,
-    // counters, off-tile lists) and restore it after each candidate, so tuning
-    // leaves the simulation state as it found it.
+    // Save everything a trial changes (points in both halves of the point
+    // buffer, cell starts, force lines, counters, off-tile lists) and restore
+    // it, with the domains' halves, after each candidate, so tuning leaves the
+    // simulation state as it found it.
     const size_t pts_bytes = sizeof(float) * h->pitch * h->nlines;
     const size_t cs_bytes = sizeof(uint32_t) * h->cs_total;
     const size_t f_bytes = sizeof(float) * h->gpitch * 2;
     const size_t c_bytes = sizeof(Counters) * (h->dom.size() + 1);
     const size_t o_bytes = sizeof(uint32_t) * h->total_capacity;
     char *save = nullptr;
-    HIPCHK(h, hipMalloc(&save, pts_bytes + cs_bytes + f_bytes + c_bytes + o_bytes));
+    HIPCHK(h, hipMalloc(&save, 2 * pts_bytes + cs_bytes + f_bytes + c_bytes + o_bytes));
     const std::vector<Domain> dom_save = h->dom;
-    float *const pts_save = h->pts;
     auto copy_all = [&](bool store) -> hipError_t {
-        if (!store && h->pts != pts_save) std::swap(h->pts, h->pts_alt);   // undo a trial sort's swap
         char *p = save;
-        void *dev[] = {h->pts, h->cs, h->gline(G_FX), h->cnt, h->offtile};
-        const size_t len[] = {pts_bytes, cs_bytes, f_bytes, c_bytes, o_bytes};
-        for (int k = 0; k < 5; k++) {
+        void *dev[] = {h->pts, h->pts + h->half_floats, h->cs, h->gline(G_FX), h->cnt, h->offtile};
+        const size_t len[] = {pts_bytes, pts_bytes, cs_bytes, f_bytes, c_bytes, o_bytes};
+        for (int k = 0; k < 6; k++) {
             hipError_t e = store ? hipMemcpyAsync(p, dev[k], len[k], hipMemcpyDeviceToDevice, h->stream)
                                  : hipMemcpyAsync(dev[k], p, len[k], hipMemcpyDeviceToDevice, h->stream);
             if (e != hipSuccess) return e;

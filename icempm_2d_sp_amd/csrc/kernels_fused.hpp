@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
         P.gx = dd.gx;
         P.gy = dd.gy;
         P.has_current = dd.has_current;
-        const Lines L{L0.base + dd.pt_base, L0.pitch};
+        const Lines L{L0.base + dd.pt_off, L0.pitch};
         const uint32_t *__restrict__ cs = cs0 + dd.cs_base;
         const size_t nb = dd.node_base;
         const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void k_offtile_pipeline(DevParams P0, StepArg 
     P.gx = dd.gx;
     P.gy = dd.gy;
     P.has_current = dd.has_current;
-    const Lines L{L0.base + dd.pt_base, L0.pitch};
+    const Lines L{L0.base + dd.pt_off, L0.pitch};
     const uint32_t *offtile = off0 + dd.off_base;
     const size_t nb = dd.node_base;
     const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,

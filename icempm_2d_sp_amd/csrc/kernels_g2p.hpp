@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_g2p_offtile(DevParams P0, StepArg S, co
     P.gx = dd.gx;
     P.gy = dd.gy;
     P.has_current = dd.has_current;
-    const Lines L{L0.base + dd.pt_base, L0.pitch};
+    const Lines L{L0.base + dd.pt_off, L0.pitch};
     const uint32_t *offtile = off0 + dd.off_base;
     const size_t nb = dd.node_base;
     const float *gm = G0.m + nb, *gpx = G0.px + nb, *gpy = G0.py + nb;

@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_p2g_offtile(DevParams P0, const DomainD
     DevParams P = P0;
     P.gx = dd.gx;
     P.gy = dd.gy;
-    const Lines L{L0.base + dd.pt_base, L0.pitch};
+    const Lines L{L0.base + dd.pt_off, L0.pitch};
     const uint32_t *offtile = off0 + dd.off_base;
     float *gm = G0.m + dd.node_base, *gpx = G0.px + dd.node_base, *gpy = G0.py + dd.node_base;
     uint32_t err = 0;

@@ -24,14 +24,19 @@ constexpr int kTileThreads = 256;
 struct DomainDesc {
     int gx, gy;            // nodes
     int has_current;
-    int pad0;
-    uint32_t pt_base;      // first point slot of the domain
     uint32_t off_base;     // first slot of the domain's off-tile list
+    uint64_t pt_off;       // first point slot of the domain from the point buffer's base (its half included)
     uint64_t node_base;    // first node of the domain in the grid lines
     uint64_t cs_base;      // first entry of the domain's cell-start array
     int ntx, nty;          // tiles per axis
     uint64_t frame_base;   // first frame of the domain (frames in tile-grid order, ty fastest)
 };
+
+// A domain's point slots moved to the other half of the point buffer (after
+// its sort): the one table entry rewritten in stream order.
+__global__ void k_set_pt_off(DomainDesc *d, uint64_t pt_off) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) d->pt_off = pt_off;
+}
 
 struct TileRef {
     uint16_t d, tx, ty, pad;
@@ -82,7 +87,7 @@ __device__ __forceinline__ TileCtx tile_ctx(const DevParams &P0, const DomainDes
     c.P.gx = dd.gx;
     c.P.gy = dd.gy;
     c.P.has_current = dd.has_current;
-    c.L = Lines{L0.base + dd.pt_base, L0.pitch};
+    c.L = Lines{L0.base + dd.pt_off, L0.pitch};
     c.cs = cs0 + dd.cs_base;
     const size_t nb = dd.node_base;
     c.G = GridLines{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
