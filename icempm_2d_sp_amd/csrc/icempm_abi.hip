@@ -81,6 +81,7 @@ struct IceHandle {
     Counters *cnt = nullptr;       // [ndom + 1]; cnt[ndom].step is the device step counter
     Counters *host_cnt = nullptr;  // pinned copy
     uint32_t *offtile = nullptr;
+    uint32_t *off_pre = nullptr;   // [ndom + 1]: prefix of the off-tile list lengths (batches)
     uint32_t *cs = nullptr;        // cell starts of every domain
     size_t cs_total = 0;
     DomainDesc *d_desc = nullptr;
@@ -281,8 +282,8 @@ using G2PFn = void (*)(DevParams, StepArg, const DomainDesc *, const TileRef *, 
                        GridLines, Counters *, uint32_t *);
 using PipeFn = void (*)(DevParams, StepArg, SurrArg, const DomainDesc *, const TileRef *, int, Lines,
                         const uint32_t *, GridLines, Counters *, uint32_t *, const float *, float *, uint32_t *);
-using OffFn = void (*)(DevParams, StepArg, const DomainDesc *, Lines, GridLines, Counters *, const uint32_t *,
-                       const float *, float *, int);
+using OffFn = void (*)(DevParams, StepArg, const DomainDesc *, int, const uint32_t *, Lines, GridLines, Counters *,
+                       const uint32_t *, const float *, float *, int);
 using ToGridFn = void (*)(const DomainDesc *, int, const float *, GridLines);
 
 struct TileKernels {
@@ -378,6 +379,7 @@ int upload_tables(IceHandle *h) {
         h->tiles_cap = tiles.size();
     }
     if (!h->d_desc) HIPCHK(h, hipMalloc(&h->d_desc, std::max(1, nd) * sizeof(DomainDesc)));
+    if (!h->off_pre) HIPCHK(h, hipMalloc(&h->off_pre, (nd + 1) * sizeof(uint32_t)));
     HIPCHK(h, hipMemcpyAsync(h->d_desc, desc.data(), nd * sizeof(DomainDesc), hipMemcpyHostToDevice, h->stream));
     if (!tiles.empty())
         HIPCHK(h, hipMemcpyAsync(h->d_tiles, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice,
@@ -475,12 +477,30 @@ void rec(IceHandle *h, int k) {
     if (t) (void)hipEventRecord(t->e[k], h->stream);
 }
 
-// Workgroups per domain of the off-tile kernels (grid-stride over the list,
-// whose length only the device knows): up to one per 2048 slots of the
+// Workgroups per domain of the unfused off-tile kernels (grid-stride over the
+// list, whose length only the device knows): up to one per 2048 slots of the
 // largest domain, so a long list (fast flows between re-sorts) is spread over
 // the whole GPU; blocks past the list's end exit after one load.
 unsigned offtile_blocks(const IceHandle *h) {
     return std::min<unsigned>(1024, blocks_for(h->max_capacity(), 2048));
+}
+
+// The pipelined off-tile pass: one grid over every domain's list (for a batch,
+// the lists' prefix first); up to one workgroup per 2048 slots of all domains.
+int launch_offtile(IceHandle *h, const TileKernels *tk, const StepArg &S, const GridLines &G, const float *Fc,
+                   float *Fn, int do_g2p) {
+    const uint32_t *pre = nullptr;
+    if (h->ndom() > 1) {
+        hipLaunchKernelGGL(k_offtile_prefix, dim3(1), dim3(256), 0, h->stream, (const Counters *)h->cnt, h->ndom(),
+                           h->off_pre);
+        LAUNCHCHK(h, "k_offtile_prefix");
+        pre = h->off_pre;
+    }
+    const unsigned blocks = std::min<unsigned>(1024, blocks_for(h->total_capacity, 2048));
+    hipLaunchKernelGGL(tk->offtile, dim3(blocks), dim3(256), 0, h->stream, h->dp, S, (const DomainDesc *)h->d_desc,
+                       h->ndom(), pre, h->lines(), G, h->cnt, (const uint32_t *)h->offtile, Fc, Fn, do_g2p);
+    LAUNCHCHK(h, "k_offtile_pipeline");
+    return ICE_OK;
 }
 
 DevParams dom_params(const IceHandle *h, int d) {
@@ -638,9 +658,8 @@ int pipeline_fill(IceHandle *h) {
                            (const float *)F, F, queue);
         LAUNCHCHK(h, "k_tile_pipeline (fill)");
     }
-    hipLaunchKernelGGL(tk->offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S0, h->d_desc,
-                       h->lines(), G, h->cnt, (const uint32_t *)h->offtile, (const float *)F, F, 0);
-    LAUNCHCHK(h, "k_offtile_pipeline (fill)");
+    int rc = launch_offtile(h, tk, S0, G, F, F, 0);
+    if (rc) return rc;
     h->frames_valid = true;
     h->last_fused = false;
     return ICE_OK;
@@ -675,9 +694,7 @@ int phase_pipelined(IceHandle *h, const StepArg &S) {
         LAUNCHCHK(h, "k_tile_pipeline");
     }
     rec(h, 4);
-    hipLaunchKernelGGL(tk->offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S, h->d_desc,
-                       h->lines(), G, h->cnt, (const uint32_t *)h->offtile, Fc, Fn, 1);
-    LAUNCHCHK(h, "k_offtile_pipeline");
+    if ((rc = launch_offtile(h, tk, S, G, Fc, Fn, 1))) return rc;
     h->fr_cur ^= 1;
     h->last_fused = true;
     for (auto &D : h->dom) D.steps_since_sort++;
@@ -1069,7 +1086,7 @@ int ice_destroy(IceHandle *h) {
     for (void *p : {(void *)h->pts, (void *)h->grid, (void *)h->status, (void *)h->cnt,
                     (void *)h->offtile, (void *)h->s_count, (void *)h->cs, (void *)h->s_key, (void *)h->s_slot,
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
-                    (void *)h->d_desc, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
+                    (void *)h->d_desc, (void *)h->off_pre, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
                     (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk, (void *)h->mlp_ipk, (void *)h->colors, (void *)h->tile_queue,
                     (void *)h->mlp_fail, (void *)h->mlp_x, (void *)h->mlp_t})
         if (p) (void)hipFree(p);

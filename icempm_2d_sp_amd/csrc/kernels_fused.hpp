@@ -617,84 +617,127 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
     }
 }
 
+// One off-tile point i of a domain (k_offtile_pipeline).
+template <int TW, int TH>
+__device__ __forceinline__ void offtile_point(const DevParams &P, const DomainDesc &dd, const Lines &L,
+                                              const GridLines &G, Counters *cnt, uint32_t i, const float *Fc,
+                                              float *Fn, int do_g2p, bool rec, float ramp, uint32_t &err) {
+    uint32_t u = L.u(ICE_IDX_UTILITY, i);
+    if (u & kDisabled) return;
+    if (u & kFresh) {
+        u &= ~kFresh;
+        L.set_u(ICE_IDX_UTILITY, i, u);
+    } else if (do_g2p) {
+        G2PState s;
+        load_g2p_state(L, i, u, s);
+        if (!stencil_inside(s.cx, s.cy, P)) {
+            err |= ICE_ERR_POINT_LEFT_AREA;
+            return;
+        }
+        const float th = L(ICE_IDX_THICKNESS, i);
+        float wx[3], wy[3];
+        weights(s.x, wx);
+        weights(s.y, wy);
+        float pvx = 0.f, pvy = 0.f;
+        M2 C{0.f, 0.f, 0.f, 0.f};
+        for (int a = -1; a <= 1; a++)
+            for (int b = -1; b <= 1; b++) {
+                const float w = wx[a + 1] * wy[b + 1];
+                float nvx, nvy, mx, my, m;
+                bool wall;
+                frame_node_velocity<TW, TH>(Fc, dd, G, P, ramp, s.cx + a, s.cy + b, nvx, nvy, mx, my, wall, m);
+                const float dx = (float)a - s.x, dy = (float)b - s.y;
+                pvx += w * nvx;
+                pvy += w * nvy;
+                const float f = P.four_h_inv * w;
+                C.a00 += f * (nvx * dx);
+                C.a01 += f * (nvx * dy);
+                C.a10 += f * (nvy * dx);
+                C.a11 += f * (nvy * dy);
+            }
+        FInv vf;
+        g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
+        if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
+        g2p_store(L, i, s, pvx, pvy, C, rec);
+        if (s.u & kDisabled) return;
+    }
+    P2GPoint q;
+    load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
+    if (!stencil_inside(q.cx, q.cy, P)) {
+        err |= ICE_ERR_POINT_LEFT_AREA;
+        return;
+    }
+    frame_scatter_global<TW, TH>(q, dd, Fn, err);
+}
+
 // The off-tile points after a pipeline pass.  do_g2p = 1 (fused substep):
 // points that escaped in an earlier substep run update + G2P from frames Fc;
 // every off-tile point then adds its P2G into Fn with global atomics.  Points
 // marked kFresh (escaped in this substep's tile pass) only need the P2G.
-// One row of workgroups per domain (blockIdx.y).
+// The domains' lists form one item space (pre: exclusive prefix of their
+// lengths, from k_offtile_prefix; null for a single domain), so a long list in
+// one domain is spread over the whole launch and a batch of empty lists costs
+// one small grid.
 template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_offtile_pipeline(DevParams P0, StepArg S, const DomainDesc *__restrict__ D,
-                                                          Lines L0, GridLines G0, Counters *cnt0,
+                                                          int ndom, const uint32_t *__restrict__ pre, Lines L0,
+                                                          GridLines G0, Counters *cnt0,
                                                           const uint32_t *__restrict__ off0, const float *Fc,
                                                           float *Fn, int do_g2p) {
-    const DomainDesc dd = D[blockIdx.y];
-    Counters *cnt = cnt0 + blockIdx.y;
-    const uint32_t n = cnt->n_esc;
-    DevParams P = P0;
-    P.gx = dd.gx;
-    P.gy = dd.gy;
-    P.has_current = dd.has_current;
-    const Lines L{L0.base + dd.pt_off, L0.pitch};
-    const uint32_t *offtile = off0 + dd.off_base;
-    const size_t nb = dd.node_base;
-    const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
-                      G0.wind_x + nb, G0.wind_y + nb, G0.status + nb};
+    const uint32_t total = pre ? pre[ndom] : cnt0[0].n_esc;
     double time = 0.0;
     bool rec = false;
-    if (do_g2p) resolve_step(S, P, time, rec);
+    if (do_g2p) resolve_step(S, P0, time, rec);
     const float ramp = current_ramp(time);
-    uint32_t err = 0;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t i = offtile[j];
-        uint32_t u = L.u(ICE_IDX_UTILITY, i);
-        if (u & kDisabled) continue;
-        if (u & kFresh) {
-            u &= ~kFresh;
-            L.set_u(ICE_IDX_UTILITY, i, u);
-        } else if (do_g2p) {
-            G2PState s;
-            load_g2p_state(L, i, u, s);
-            if (!stencil_inside(s.cx, s.cy, P)) {
-                err |= ICE_ERR_POINT_LEFT_AREA;
-                continue;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+        // item j's domain: the last d with pre[d] <= j (empty domains skipped)
+        int d = 0;
+        if (pre) {
+            int hi = ndom - 1;
+            while (d < hi) {
+                const int mid = (d + hi + 1) >> 1;
+                if (pre[mid] <= j) d = mid;
+                else hi = mid - 1;
             }
-            const float th = L(ICE_IDX_THICKNESS, i);
-            float wx[3], wy[3];
-            weights(s.x, wx);
-            weights(s.y, wy);
-            float pvx = 0.f, pvy = 0.f;
-            M2 C{0.f, 0.f, 0.f, 0.f};
-            for (int a = -1; a <= 1; a++)
-                for (int b = -1; b <= 1; b++) {
-                    const float w = wx[a + 1] * wy[b + 1];
-                    float nvx, nvy, mx, my, m;
-                    bool wall;
-                    frame_node_velocity<TW, TH>(Fc, dd, G, P, ramp, s.cx + a, s.cy + b, nvx, nvy, mx, my, wall, m);
-                    const float dx = (float)a - s.x, dy = (float)b - s.y;
-                    pvx += w * nvx;
-                    pvy += w * nvy;
-                    const float f = P.four_h_inv * w;
-                    C.a00 += f * (nvx * dx);
-                    C.a01 += f * (nvx * dy);
-                    C.a10 += f * (nvy * dx);
-                    C.a11 += f * (nvy * dy);
-                }
-            FInv vf;
-            g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
-            if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
-            g2p_store(L, i, s, pvx, pvy, C, rec);
-            if (s.u & kDisabled) continue;
-            u = s.u;
         }
-        P2GPoint q;
-        load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
-        if (!stencil_inside(q.cx, q.cy, P)) {
-            err |= ICE_ERR_POINT_LEFT_AREA;
-            continue;
-        }
-        frame_scatter_global<TW, TH>(q, dd, Fn, err);
+        const DomainDesc dd = D[d];
+        Counters *cnt = cnt0 + d;
+        DevParams P = P0;
+        P.gx = dd.gx;
+        P.gy = dd.gy;
+        P.has_current = dd.has_current;
+        const Lines L{L0.base + dd.pt_off, L0.pitch};
+        const size_t nb = dd.node_base;
+        const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
+                          G0.wind_x + nb, G0.wind_y + nb, G0.status + nb};
+        uint32_t err = 0;
+        const uint32_t i = off0[dd.off_base + (j - (pre ? pre[d] : 0u))];
+        offtile_point<TW, TH>(P, dd, L, G, cnt, i, Fc, Fn, do_g2p, rec, ramp, err);
+        if (err) atomicOr(&cnt->err, err);   // rare: per item, to the item's domain
     }
-    flush_err(cnt, err);
+}
+
+// Exclusive prefix of the domains' off-tile list lengths, pre[0..ndom], one
+// workgroup (the item space of k_offtile_pipeline for a batch).
+__global__ __launch_bounds__(256) void k_offtile_prefix(const Counters *__restrict__ cnt, int ndom, uint32_t *pre) {
+    __shared__ uint32_t s[256];
+    uint32_t carry = 0;
+    for (int base = 0; base < ndom; base += 256) {
+        const int d = base + (int)threadIdx.x;
+        const uint32_t v = d < ndom ? cnt[d].n_esc : 0u;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const uint32_t t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (d < ndom) pre[d] = carry + s[threadIdx.x] - v;
+        carry += s[255];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pre[ndom] = carry;
 }
 
 // Node sums of domain d from frames F into the grid lines (m, px, py).
