@@ -449,7 +449,10 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
             LAUNCHCHK(h, "k_set_pt_off");
         }
     }
-    drop_graph(h);   // a captured graph holds the old point buffer
+    // the sorted paths' graphs read the point base, the domain table, the cell
+    // starts and the counters from device memory and stay valid; the naive
+    // path captures each domain's point range and count by value
+    if (h->naive() || drop) drop_graph(h);
     D.n = n_out;
     Counters *c = h->dcnt(d);
     if (drop && (rc = launch_fill_u32(h, reinterpret_cast<uint32_t *>(&c->disabled), 1, 0u))) return rc;
