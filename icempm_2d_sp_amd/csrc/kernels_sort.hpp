@@ -98,27 +98,40 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__r
 }
 
 // Points whose cell word lies outside the grid cannot be binned; they join
-// the disabled tail (and raise the left-area error bit).
+// the disabled tail (and raise the left-area error bit).  Points arrive
+// nearly sorted, so neighbouring lanes mostly share a cell: each run of equal
+// keys in a wave takes its slots with one atomic (the slot order inside a
+// cell does not matter, k_sort_cells orders each cell by input index).
 __global__ __launch_bounds__(256) void k_sort_keys(Lines L, uint32_t n, int gx, int gy, uint32_t NC,
                                                    uint32_t *__restrict__ count, uint32_t *__restrict__ key,
                                                    uint32_t *__restrict__ slot, uint32_t *__restrict__ dflag,
                                                    Counters *cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t u = L.u(ICE_IDX_UTILITY, i);
-    const uint32_t c0 = L.u(ICE_IDX_CELL, i);
-    const bool outside = (int)(c0 & 0xffffu) >= gx || (int)(c0 >> 16) >= gy;
-    if (outside && !(u & kDisabled)) atomicOr(&cnt->err, ICE_ERR_POINT_LEFT_AREA);
-    if ((u & kDisabled) || outside) {
-        key[i] = NC;
-        slot[i] = 0;
-        dflag[i] = 1;
-    } else {
+    const int lane = (int)(threadIdx.x & 63);
+    bool binned = false;
+    uint32_t k = 0xffffffffu;   // never a cell key: lanes not binned start runs of their own
+    if (i < n) {
+        const uint32_t u = L.u(ICE_IDX_UTILITY, i);
         const uint32_t c = L.u(ICE_IDX_CELL, i);
-        const uint32_t k = (c & 0xffffu) * (uint32_t)gy + (c >> 16);
-        key[i] = k;
-        slot[i] = atomicAdd(&count[k], 1u);
-        dflag[i] = 0;
+        const bool outside = (int)(c & 0xffffu) >= gx || (int)(c >> 16) >= gy;
+        if (outside && !(u & kDisabled)) atomicOr(&cnt->err, ICE_ERR_POINT_LEFT_AREA);
+        binned = !(u & kDisabled) && !outside;
+        if (binned) k = (c & 0xffffu) * (uint32_t)gy + (c >> 16);
+    }
+    const uint32_t prev = __shfl_up(k, 1, 64);
+    const bool head = lane == 0 || prev != k;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);   // lanes 0..lane
+    const int first = 63 - __clzll(heads & upto);                      // this run's head lane
+    const uint64_t later = heads & ~upto;
+    const int end = later ? __ffsll((unsigned long long)later) - 1 : 64;   // next run's head lane
+    uint32_t base = 0;
+    if (head && binned) base = atomicAdd(&count[k], (uint32_t)(end - lane));
+    base = __shfl(base, first, 64);
+    if (i < n) {
+        key[i] = binned ? k : NC;
+        slot[i] = binned ? base + (uint32_t)(lane - first) : 0u;
+        dflag[i] = binned ? 0u : 1u;
     }
 }
 
