@@ -113,6 +113,8 @@ def main() -> int:
     ap.add_argument("--naive", action="store_true", help="reference launch structure (unsorted, atomics)")
     ap.add_argument("--unfused", action="store_true", help="separate P2G and update + G2P tile kernels")
     ap.add_argument("--persistent", action="store_true", help="tile passes as persistent launches (work queues)")
+    ap.add_argument("--resort-drift", type=int, default=None,
+                    help="re-sort a domain when this %% of its points left their sorted cell (0 = off; library default 30)")
     ap.add_argument("--scale", type=float, default=1.0, help="linear domain scale (1.0 = 4096x2048)")
     ap.add_argument("--workload", choices=["icefield", "batch64", "surrogate", "frame"], default="icefield",
                     help="icefield: one 4096x2048 domain (the headline); batch64: 64 uneven domains; "
@@ -159,18 +161,22 @@ def main() -> int:
     tw, th = (int(v) for v in args.tile.split("x"))
     flags = scene.params.flags | (abi.FLAG_NAIVE if args.naive else 0)
     pipelined = not (args.naive or args.unfused)
-    sim = Simulation(scene, flags=flags, tile=(tw, th),
-                     options={"pipelined": int(pipelined), "persistent": int(args.persistent)})
+    opts = {"pipelined": int(pipelined), "persistent": int(args.persistent)}
+    if args.resort_drift is not None:
+        opts["resort_drift"] = args.resort_drift
+    sim = Simulation(scene, flags=flags, tile=(tw, th), options=opts)
     n_active = sim.info().n_points
     print(f"[bench] uploaded and sorted; warmup {args.warmup}", file=sys.stderr, flush=True)
     sim.run(0, args.warmup, graph=False)
     sim.sync()
 
     # timed region: graph replays of whole substeps
+    sorts0 = sim.info().n_sorts
     t0 = time.perf_counter()
     sim.run(args.warmup, args.steps, graph=True)
     sim.sync()
     wall = time.perf_counter() - t0
+    sorts_timed = sim.info().n_sorts - sorts0
     err, dis = _err_dis(sim)
     print(f"[bench] timed {args.steps} substeps in {wall:.3f}s; err {err:#x}", file=sys.stderr, flush=True)
 
@@ -242,6 +248,7 @@ def main() -> int:
             "points": n_active, "nodes": nodes, "tile": args.tile,
             "path": "naive" if args.naive else ("sorted-unfused" if args.unfused else
                                                 ("sorted-pipelined-persistent" if args.persistent else "sorted-pipelined")),
+            "sorts_in_timed_region": sorts_timed,
             "error_word": err,
         },
         "roofline": {
@@ -274,15 +281,20 @@ def bench_batch(args) -> int:
     print(f"[bench] {len(doms)} domains, {sum(d.n for d in doms)} points, generated in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     steps = max(16, (args.steps // 16) * 16)
-    b = BatchSimulation(doms, options={"persistent": int(args.persistent)})
+    opts = {"persistent": int(args.persistent)}
+    if args.resort_drift is not None:
+        opts["resort_drift"] = args.resort_drift
+    b = BatchSimulation(doms, options=opts)
     tile, tune_ms = b.autotune(2)
     n_active = b.n_points()
     b.run(0, 16, graph=True)
     b.sync()
+    sorts0 = sum(b.info(d).n_sorts for d in range(len(doms)))
     t0 = time.perf_counter()
     b.run(16, steps, graph=True)
     b.sync()
     wall_b = time.perf_counter() - t0
+    sorts_b = sum(b.info(d).n_sorts for d in range(len(doms))) - sorts0
     b.close()
     print(f"[bench] batched: {steps} substeps in {wall_b:.3f}s (tile {tile})", file=sys.stderr, flush=True)
     # one domain at a time, each for the same substeps
@@ -304,7 +316,8 @@ def bench_batch(args) -> int:
         "n_gpus": 1, "steps": steps, "warmup": 16, "ms_per_step": wall_b * 1000 / steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batch64_uneven: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, 4 points/cell, "
-                               "per-domain wind 0.05-0.3 Pa", "points": n_active, "tile": f"{tile[0]}x{tile[1]}"},
+                               "per-domain wind 0.05-0.3 Pa", "points": n_active, "tile": f"{tile[0]}x{tile[1]}",
+                   "sorts_in_timed_region": sorts_b},
         "sequential_single_domain": {"value": n_active * steps / wall_s, "ms_per_step": wall_s * 1000 / steps},
         "batched_over_sequential": wall_s / wall_b,
     }

@@ -113,6 +113,7 @@ struct IceHandle {
     bool last_fused = false;       // the last substep ran pipelined: its grid is fr[fr_cur ^ 1]
     bool fused_opt = true;
     bool persistent = false;          // option "persistent": tile passes as persistent launches
+    int resort_drift = 30;            // option "resort_drift": re-sort when this % of points left their cell
     uint32_t *tile_queue = nullptr;   // their per-XCD tile counters
     int n_cu = 0;
     // graph
@@ -1179,6 +1180,9 @@ int ice_set_option(IceHandle *h, const char *name, int value) {
         if (value != 0 && value != 1) { h->err = "pipelined must be 0 or 1"; return ICE_E_ARG; }
         h->fused_opt = value != 0;
         frames_stale(h);
+    } else if (std::strcmp(name, "resort_drift") == 0) {
+        if (value < 0 || value > 100) { h->err = "resort_drift must be 0..100 (0: off)"; return ICE_E_ARG; }
+        h->resort_drift = value;
     } else if (std::strcmp(name, "persistent") == 0) {
         if (value != 0 && value != 1) { h->err = "persistent must be 0 or 1"; return ICE_E_ARG; }
         HIPCHK(h, hipSetDevice(h->device));
@@ -1255,6 +1259,10 @@ int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
     int done = 0;
     while (done < nsteps) {
         const int chunk = std::min(kChunk, nsteps - done);
+        if (!h->naive()) {   // the chunk's last substep counts the points off their sorted cell
+            hipLaunchKernelGGL(k_clear_drift, dim3(1), dim3(256), 0, h->stream, h->cnt, h->ndom());
+            LAUNCHCHK(h, "k_clear_drift");
+        }
         if (use_graph && chunk == kChunk && !h->timing) {
             if (!h->graph_exec || h->graph_chunk != chunk || h->graph_fr0 != h->fr_cur) {
                 if ((rc = build_graph(h, chunk))) return rc;
@@ -1272,11 +1280,16 @@ int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
         }
         done += chunk;
         if (!h->naive()) {
-            // re-sort domains whose off-tile list grew past max(256, n/256)
+            // re-sort domains whose off-tile list grew past max(256, n/256), and
+            // (pipelined path) those where resort_drift % of the points sit off
+            // their sorted cell: those escape in bulk within the next chunk
             if ((rc = read_counters(h))) return rc;
             for (int d = 0; d < h->ndom(); d++) {
-                const uint32_t limit = std::max<uint32_t>(256, h->dom[d].n / 256);
-                if (h->host_cnt[d].n_esc > limit && (rc = device_sort(h, d, false, nullptr))) return rc;
+                const Domain &D = h->dom[d];
+                const uint32_t limit = std::max<uint32_t>(256, D.n / 256);
+                const bool drifted = h->resort_drift > 0 &&
+                                     (uint64_t)h->host_cnt[d].n_drift * 100 > (uint64_t)D.n * h->resort_drift;
+                if ((h->host_cnt[d].n_esc > limit || drifted) && (rc = device_sort(h, d, false, nullptr))) return rc;
             }
         }
     }

@@ -495,6 +495,10 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                                 stage = true;
                             }
                         }
+                        if (S.complete) {   // points off their sorted cell at a chunk's end (re-sort policy)
+                            const uint64_t bm = __ballot(stage && (s.u & kMoved));
+                            if (lane == 0 && bm) atomicAdd(&cnt->n_drift, (uint32_t)__popcll(bm));
+                        }
                         stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
                     }
                 } else {
@@ -715,6 +719,11 @@ __global__ __launch_bounds__(256) void k_offtile_pipeline(DevParams P0, StepArg 
         offtile_point<TW, TH>(P, dd, L, G, cnt, i, Fc, Fn, do_g2p, rec, ramp, err);
         if (err) atomicOr(&cnt->err, err);   // rare: per item, to the item's domain
     }
+}
+
+// Clears every domain's drift count before a run chunk.
+__global__ __launch_bounds__(256) void k_clear_drift(Counters *cnt, int ndom) {
+    for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < ndom; d += gridDim.x * blockDim.x) cnt[d].n_drift = 0;
 }
 
 // Exclusive prefix of the domains' off-tile list lengths, pre[0..ndom], one

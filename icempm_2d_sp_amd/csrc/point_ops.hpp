@@ -36,7 +36,9 @@ struct Counters {
     int32_t disabled;    // gpu_disabled_points_count (kernels.cuh:29)
     uint32_t n_esc;      // points on the off-tile list
     int32_t step;        // device step counter for graph replays
-    uint32_t pad[4];
+    uint32_t pad[2];     // learned projection: accepted, fallbacks
+    uint32_t n_drift;    // points off their sorted cell, counted on a run chunk's last substep
+    uint32_t pad3;
 };
 
 // The SoA lines of a point range.  Slot i of a line is addressed as the

@@ -217,7 +217,12 @@ int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
  * workgroup-to-tile order; "pipelined" (0 or 1, default 1) selects the
  * pipelined substep of ice_substep / ice_run; "persistent" (0 or 1, default
  * 0) runs its tile passes (without the surrogate) as persistent launches, as
- * many workgroups as fit on the device pulling tiles from per-XCD queues. */
+ * many workgroups as fit on the device pulling tiles from per-XCD queues;
+ * "resort_drift" (0..100, default 30; 0 turns it off): between ice_run's
+ * 16-substep chunks a domain is re-sorted when its off-tile list exceeds
+ * max(256, n/256) points and also, on the pipelined path, when more than this
+ * percentage of its points sit in a cell other than the one they were sorted
+ * into (they leave the tile pass in bulk once they move again). */
 int ice_set_option(IceHandle *h, const char *name, int value);
 /* Tile autotuner (no reference counterpart): times `nsteps` substeps for each
  * tile shape from 8x8 to 64x32, restoring the simulation state after each

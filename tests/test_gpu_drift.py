@@ -75,3 +75,37 @@ def test_diagonal_drift_stays_in_the_tile_pass(pipelined):
         ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
     ])
     sim.close()
+
+
+@pytest.mark.parametrize("pct", [30, 0])
+def test_drift_resort_policy_between_run_chunks(pct):
+    """ice_run over two 16-substep chunks of a uniform flow of 0.9 cells in 32
+    substeps (no point crosses two boundaries along one axis, so none goes
+    off-tile): after the first chunk ~70 % of the points sit off their sorted
+    cell, so with resort_drift = 30 the domain re-sorts between chunks; with
+    the rule off it never does.  Both match the oracle."""
+    _print("test_drift_resort_policy_between_run_chunks pct", pct)
+    sc = uniform_drift_scene(32, cells=0.9)
+    sim = Simulation(sc, options={"resort_drift": pct})
+    sim.run(0, 0, graph=True)                      # the initial sort
+    n0 = sim.info().n_sorts
+    sim.run(0, 32, graph=True)
+    info = sim.info()
+    pts, _, err, dis = sim.download()
+    o = oracle.Oracle.from_scene(sc, 64)
+    o.run(0, 16)
+    half = float(np.mean(u32(by_point(o.points()), abi.IDX_CELL) != u32(by_point(sc.soa), abi.IDX_CELL)))
+    o.run(16, 16)
+    _print("sorts", info.n_sorts - n0, "off-tile", info.n_offtile, "off their cell after 16", half, "error", err)
+    assert half > 0.5
+    if pct:
+        assert info.n_sorts - n0 >= 1
+    else:
+        assert info.n_sorts == n0 and info.n_offtile == 0
+    assert err == o.error == 0 and dis == o.disabled == 0
+    compare_points(pts, o.points(), fields=[
+        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
+        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
+    ])
+    sim.close()
