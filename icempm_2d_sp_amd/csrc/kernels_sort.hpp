@@ -4,11 +4,13 @@
 // removes disabled points and sorts the rest by ProxyPoint::getCellIndex =
 // x*GridY + y (proxypoint2d.cpp:117-123).  Here, per sort:
 //   1. key per point (disabled points go to bucket NC = GridX*GridY) and a slot
-//      inside its cell from an atomic counter,
+//      inside its cell from the cell's counter (one atomic per run of equal
+//      keys in a wave),
 //   2. exclusive scans: cell counts -> cs (cell start), disabled flags -> rank,
 //   3. scatter slot indices, then sort each cell's short index run ascending,
 //      which is exactly the stable order (ties keep input order),
-//   4. gather every SoA line into the second buffer in the new order.
+//   4. gather every SoA line into the domain's range of the other half of the
+//      point buffer, in the new order.
 // Disabled points either follow the sorted ones (in input order) or are
 // dropped, as the caller asks.
 #pragma once
