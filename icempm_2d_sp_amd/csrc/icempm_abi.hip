@@ -445,19 +445,18 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
                            h->nlines, n_out, h->s_order, perm_dev);
         LAUNCHCHK(h, "k_sort_gather");
         D.half ^= 1;
-        if (h->d_desc && !h->tables_dirty) {   // else the next table upload writes it
-            hipLaunchKernelGGL(k_set_pt_off, dim3(1), dim3(64), 0, h->stream, h->d_desc + d, to);
-            LAUNCHCHK(h, "k_set_pt_off");
-        }
     }
+    // the table entry (else the next table upload writes it), the off-tile
+    // list, the disabled count of a dropping sort
+    DomainDesc *entry = (n_out && h->d_desc && !h->tables_dirty) ? h->d_desc + d : nullptr;
+    hipLaunchKernelGGL(k_sort_done, dim3(1), dim3(64), 0, h->stream, entry, h->pt_off(d, D.half), h->dcnt(d),
+                       drop ? 1 : 0);
+    LAUNCHCHK(h, "k_sort_done");
     // the sorted paths' graphs read the point base, the domain table, the cell
     // starts and the counters from device memory and stay valid; the naive
     // path captures each domain's point range and count by value
     if (h->naive() || drop) drop_graph(h);
     D.n = n_out;
-    Counters *c = h->dcnt(d);
-    if (drop && (rc = launch_fill_u32(h, reinterpret_cast<uint32_t *>(&c->disabled), 1, 0u))) return rc;
-    if ((rc = launch_fill_u32(h, &c->n_esc, 1, 0u))) return rc;
     if (drop) HIPCHK(h, hipStreamSynchronize(h->stream));   // ice_sort's caller reads the result at once
     D.sorted = true;
     D.steps_since_sort = 0;

@@ -32,10 +32,16 @@ struct DomainDesc {
     uint64_t frame_base;   // first frame of the domain (frames in tile-grid order, ty fastest)
 };
 
-// A domain's point slots moved to the other half of the point buffer (after
-// its sort): the one table entry rewritten in stream order.
-__global__ void k_set_pt_off(DomainDesc *d, uint64_t pt_off) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) d->pt_off = pt_off;
+// The end of a domain's sort, in stream order: its point slots moved to the
+// other half of the point buffer (the table entry, when one is given), its
+// off-tile list emptied, and its disabled count cleared when the sort
+// dropped the disabled points.
+__global__ void k_sort_done(DomainDesc *d, uint64_t pt_off, Counters *c, int clear_disabled) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (d) d->pt_off = pt_off;
+        c->n_esc = 0;
+        if (clear_disabled) c->disabled = 0;
+    }
 }
 
 struct TileRef {
