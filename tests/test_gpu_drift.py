@@ -87,6 +87,8 @@ def test_drift_resort_policy_between_run_chunks(pct):
     _print("test_drift_resort_policy_between_run_chunks pct", pct)
     sc = uniform_drift_scene(32, cells=0.9)
     sim = Simulation(sc, options={"resort_drift": pct})
+    for bad in (-1, 101):                          # rejected, the setting kept
+        assert sim.lib.ice_set_option(sim.h, b"resort_drift", bad) == -1
     sim.run(0, 0, graph=True)                      # the initial sort
     n0 = sim.info().n_sorts
     sim.run(0, 32, graph=True)
