@@ -1,0 +1,43 @@
+"""bench.py's contract, on the CPU: the roofline's algorithmic bytes are the
+two-pass substep's per-unit figure (168 B per point with the cohesion line,
+164 B without, plus 33 B per node), the measured HBM traffic comes from the
+committed counter summary, and a multi-GPU request exits with a message
+instead of running."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_algorithmic_bytes_are_the_two_pass_substep():
+    n, nodes = 31_880_000, 4096 * 2048
+    B = bench.algorithmic_bytes(n, nodes, True)
+    assert B["substep"] == 168 * n + 33 * nodes
+    assert B["p2g"] == 64 * n + 12 * nodes               # 16 point slots read, m/px/py written
+    assert B["g2p"] == (44 + 60) * n + 21 * nodes         # 11 slots read, 15 written; node reads
+    assert bench.algorithmic_bytes(n, nodes, False)["substep"] == 164 * n + 33 * nodes
+    # the 4096x2048 figure: 5.632 GB per substep, 0.704 ms at 8 TB/s
+    assert B["substep"] / 1e9 == pytest.approx(5.632, abs=1e-3)
+    # the pipelined design's own bytes are fewer: reads 44 B, writes 24 B (+24 B on
+    # chunk-completing substeps), 37 B per node
+    own = bench.algorithmic_bytes(n, nodes, True, complete_frac=0.0)["pipelined"]
+    assert own == (44 + 24) * n + 37 * nodes
+
+
+def test_traffic_comes_from_the_committed_counter_summary():
+    t = bench.measured_traffic("k_tile_pipeline", "16x32", "4096x2048")
+    assert t is not None and 2.0e9 < t < 3.0e9
+    assert bench.measured_traffic("k_tile_pipeline", "64x64", "4096x2048") is None
+
+
+def test_multi_gpu_request_exits_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and not r.stdout.strip()
