@@ -480,28 +480,36 @@ void rec(IceHandle *h, int k) {
     if (t) (void)hipEventRecord(t->e[k], h->stream);
 }
 
-// Workgroups per domain of the unfused off-tile kernels (grid-stride over the
-// list, whose length only the device knows): up to one per 2048 slots of the
-// largest domain, so a long list (fast flows between re-sorts) is spread over
-// the whole GPU; blocks past the list's end exit after one load.
+// The off-tile kernels walk every domain's list as one item space (grid-stride
+// over a length only the device knows): for a batch, the lists' prefix first.
+// Up to one workgroup per 2048 slots of all domains, so a long list (fast
+// flows between re-sorts) is spread over the whole GPU; workgroups past the
+// end exit after one load.
 unsigned offtile_blocks(const IceHandle *h) {
-    return std::min<unsigned>(1024, blocks_for(h->max_capacity(), 2048));
+    return std::min<unsigned>(1024, blocks_for(h->total_capacity, 2048));
 }
 
-// The pipelined off-tile pass: one grid over every domain's list (for a batch,
-// the lists' prefix first); up to one workgroup per 2048 slots of all domains.
-int launch_offtile(IceHandle *h, const TileKernels *tk, const StepArg &S, const GridLines &G, const float *Fc,
-                   float *Fn, int do_g2p) {
-    const uint32_t *pre = nullptr;
+int offtile_prefix(IceHandle *h, const uint32_t *&pre) {
+    pre = nullptr;
+    if (!h->d_desc || !h->off_pre) { h->err = "domain tables not uploaded"; return ICE_E_STATE; }
     if (h->ndom() > 1) {
         hipLaunchKernelGGL(k_offtile_prefix, dim3(1), dim3(256), 0, h->stream, (const Counters *)h->cnt, h->ndom(),
                            h->off_pre);
         LAUNCHCHK(h, "k_offtile_prefix");
         pre = h->off_pre;
     }
-    const unsigned blocks = std::min<unsigned>(1024, blocks_for(h->total_capacity, 2048));
-    hipLaunchKernelGGL(tk->offtile, dim3(blocks), dim3(256), 0, h->stream, h->dp, S, (const DomainDesc *)h->d_desc,
-                       h->ndom(), pre, h->lines(), G, h->cnt, (const uint32_t *)h->offtile, Fc, Fn, do_g2p);
+    return ICE_OK;
+}
+
+// The pipelined off-tile pass.
+int launch_offtile(IceHandle *h, const TileKernels *tk, const StepArg &S, const GridLines &G, const float *Fc,
+                   float *Fn, int do_g2p) {
+    const uint32_t *pre;
+    int rc = offtile_prefix(h, pre);
+    if (rc) return rc;
+    hipLaunchKernelGGL(tk->offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, h->dp, S,
+                       (const DomainDesc *)h->d_desc, h->ndom(), pre, h->lines(), G, h->cnt,
+                       (const uint32_t *)h->offtile, Fc, Fn, do_g2p);
     LAUNCHCHK(h, "k_offtile_pipeline");
     return ICE_OK;
 }
@@ -551,8 +559,11 @@ int phase_p2g(IceHandle *h) {
     }
     rec(h, 1);
     if (!h->det()) {   // the deterministic tile kernel adds the off-tile points itself
-        hipLaunchKernelGGL(k_p2g_offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp,
-                           h->d_desc, h->lines(), G, h->cnt, (const uint32_t *)h->offtile);
+        const uint32_t *pre;
+        if ((rc = offtile_prefix(h, pre))) return rc;
+        hipLaunchKernelGGL(k_p2g_offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, h->dp,
+                           (const DomainDesc *)h->d_desc, h->ndom(), pre, h->lines(), G, h->cnt,
+                           (const uint32_t *)h->offtile);
         LAUNCHCHK(h, "k_p2g_offtile");
     }
     rec(h, 2);
@@ -585,9 +596,17 @@ int phase_g2p(IceHandle *h, const StepArg &S) {
         rec(h, 4);
         return ICE_OK;
     }
+    if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
+    for (const auto &D : h->dom)
+        if (!D.sorted) { h->err = "the sorted path's G2P needs the sorted points of a preceding P2G"; return ICE_E_STATE; }
+    int rc = upload_tables(h);   // no work once a P2G has uploaded them
+    if (rc) return rc;
     const GridLines G = grid_lines(h);
-    hipLaunchKernelGGL(k_g2p_offtile, dim3(offtile_blocks(h), h->ndom()), dim3(256), 0, h->stream, h->dp, S, h->d_desc,
-                       h->lines(), G, h->cnt, (const uint32_t *)h->offtile);
+    const uint32_t *pre;
+    if ((rc = offtile_prefix(h, pre))) return rc;
+    hipLaunchKernelGGL(k_g2p_offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, h->dp, S,
+                       (const DomainDesc *)h->d_desc, h->ndom(), pre, h->lines(), G, h->cnt,
+                       (const uint32_t *)h->offtile);
     LAUNCHCHK(h, "k_g2p_offtile");
     rec(h, 3);
     if (h->ntiles) {

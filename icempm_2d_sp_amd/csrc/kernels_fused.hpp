@@ -688,22 +688,13 @@ __global__ __launch_bounds__(256) void k_offtile_pipeline(DevParams P0, StepArg 
                                                           GridLines G0, Counters *cnt0,
                                                           const uint32_t *__restrict__ off0, const float *Fc,
                                                           float *Fn, int do_g2p) {
-    const uint32_t total = pre ? pre[ndom] : cnt0[0].n_esc;
+    const uint32_t total = offtile_total(pre, ndom, cnt0);
     double time = 0.0;
     bool rec = false;
     if (do_g2p) resolve_step(S, P0, time, rec);
     const float ramp = current_ramp(time);
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
-        // item j's domain: the last d with pre[d] <= j (empty domains skipped)
-        int d = 0;
-        if (pre) {
-            int hi = ndom - 1;
-            while (d < hi) {
-                const int mid = (d + hi + 1) >> 1;
-                if (pre[mid] <= j) d = mid;
-                else hi = mid - 1;
-            }
-        }
+        const int d = offtile_domain(pre, ndom, j);
         const DomainDesc dd = D[d];
         Counters *cnt = cnt0 + d;
         DevParams P = P0;

@@ -114,37 +114,38 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiles(DevParams P0, StepAr
 
 // G2P of the off-tile points, node velocities recomputed from the P2G output
 // (the same function the tiles evaluate).  Runs before k_g2p_tiles so points
-// that the tile G2P appends to the list are not processed twice.  One row of
-// workgroups per domain (blockIdx.y).
+// that the tile G2P appends to the list are not processed twice.  All
+// domains' lists as one item space (offtile_domain).
 __global__ __launch_bounds__(256) void k_g2p_offtile(DevParams P0, StepArg S, const DomainDesc *__restrict__ D,
-                                                     Lines L0, GridLines G0, Counters *cnt0,
+                                                     int ndom, const uint32_t *__restrict__ pre, Lines L0,
+                                                     GridLines G0, Counters *cnt0,
                                                      const uint32_t *__restrict__ off0) {
-    const DomainDesc dd = D[blockIdx.y];
-    Counters *cnt = cnt0 + blockIdx.y;
-    const uint32_t n = cnt->n_esc;
-    DevParams P = P0;
-    P.gx = dd.gx;
-    P.gy = dd.gy;
-    P.has_current = dd.has_current;
-    const Lines L{L0.base + dd.pt_off, L0.pitch};
-    const uint32_t *offtile = off0 + dd.off_base;
-    const size_t nb = dd.node_base;
-    const float *gm = G0.m + nb, *gpx = G0.px + nb, *gpy = G0.py + nb;
-    const float *ccx = G0.cur_x + nb, *ccy = G0.cur_y + nb, *wwx = G0.wind_x + nb, *wwy = G0.wind_y + nb;
-    const uint8_t *st = G0.status + nb;
+    const uint32_t total = offtile_total(pre, ndom, cnt0);
     double time;
     bool rec;
-    resolve_step(S, P, time, rec);
+    resolve_step(S, P0, time, rec);
     const float ramp = current_ramp(time);
-    uint32_t err = 0;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t i = offtile[j];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+        const int d = offtile_domain(pre, ndom, j);
+        const DomainDesc dd = D[d];
+        Counters *cnt = cnt0 + d;
+        DevParams P = P0;
+        P.gx = dd.gx;
+        P.gy = dd.gy;
+        P.has_current = dd.has_current;
+        const Lines L{L0.base + dd.pt_off, L0.pitch};
+        const size_t nb = dd.node_base;
+        const float *gm = G0.m + nb, *gpx = G0.px + nb, *gpy = G0.py + nb;
+        const float *ccx = G0.cur_x + nb, *ccy = G0.cur_y + nb, *wwx = G0.wind_x + nb, *wwy = G0.wind_y + nb;
+        const uint8_t *st = G0.status + nb;
+        const uint32_t i = off0[dd.off_base + (j - (pre ? pre[d] : 0u))];
         const uint32_t u = L.u(ICE_IDX_UTILITY, i);
         if (u & kDisabled) continue;
+        uint32_t err = 0;
         G2PState s;
         load_g2p_state(L, i, u, s);
         if (!stencil_inside(s.cx, s.cy, P)) {
-            err |= ICE_ERR_POINT_LEFT_AREA;
+            atomicOr(&cnt->err, ICE_ERR_POINT_LEFT_AREA);
             continue;
         }
         const float th = L(ICE_IDX_THICKNESS, i);
@@ -176,8 +177,8 @@ __global__ __launch_bounds__(256) void k_g2p_offtile(DevParams P0, StepArg S, co
         g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
         g2p_store(L, i, s, pvx, pvy, C, rec);
+        if (err) atomicOr(&cnt->err, err);   // rare: per item, to the item's domain
     }
-    flush_err(cnt, err);
 }
 
 // Node velocities of domain d for time t into (vx, vy), for inspection.

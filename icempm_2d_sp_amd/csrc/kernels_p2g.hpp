@@ -252,28 +252,28 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const 
     flush_err(cnt, err);
 }
 
-// P2G of the off-tile points: global atomics on top of the tile stores.  One
-// row of workgroups per domain (blockIdx.y).
-__global__ __launch_bounds__(256) void k_p2g_offtile(DevParams P0, const DomainDesc *__restrict__ D, Lines L0,
-                                                     GridLines G0, Counters *cnt0, const uint32_t *__restrict__ off0) {
-    const DomainDesc dd = D[blockIdx.y];
-    Counters *cnt = cnt0 + blockIdx.y;
-    const uint32_t n = cnt->n_esc;
-    DevParams P = P0;
-    P.gx = dd.gx;
-    P.gy = dd.gy;
-    const Lines L{L0.base + dd.pt_off, L0.pitch};
-    const uint32_t *offtile = off0 + dd.off_base;
-    float *gm = G0.m + dd.node_base, *gpx = G0.px + dd.node_base, *gpy = G0.py + dd.node_base;
-    uint32_t err = 0;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t i = offtile[j];
+// P2G of the off-tile points: global atomics on top of the tile stores.  All
+// domains' lists as one item space (offtile_domain).
+__global__ __launch_bounds__(256) void k_p2g_offtile(DevParams P0, const DomainDesc *__restrict__ D, int ndom,
+                                                     const uint32_t *__restrict__ pre, Lines L0, GridLines G0,
+                                                     Counters *cnt0, const uint32_t *__restrict__ off0) {
+    const uint32_t total = offtile_total(pre, ndom, cnt0);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+        const int d = offtile_domain(pre, ndom, j);
+        const DomainDesc dd = D[d];
+        DevParams P = P0;
+        P.gx = dd.gx;
+        P.gy = dd.gy;
+        const Lines L{L0.base + dd.pt_off, L0.pitch};
+        float *gm = G0.m + dd.node_base, *gpx = G0.px + dd.node_base, *gpy = G0.py + dd.node_base;
+        const uint32_t i = off0[dd.off_base + (j - (pre ? pre[d] : 0u))];
         const uint32_t u = L.u(ICE_IDX_UTILITY, i);
         if (u & kDisabled) continue;
+        uint32_t err = 0;
         P2GPoint q;
         load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
         if (!stencil_inside(q.cx, q.cy, P)) {
-            err |= ICE_ERR_POINT_LEFT_AREA;
+            atomicOr(&cnt0[d].err, ICE_ERR_POINT_LEFT_AREA);
             continue;
         }
         float wx[3], wy[3];
@@ -290,8 +290,8 @@ __global__ __launch_bounds__(256) void k_p2g_offtile(DevParams P0, const DomainD
                 if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
                 if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
             }
+        if (err) atomicOr(&cnt0[d].err, err);   // rare: per item, to the item's domain
     }
-    flush_err(cnt, err);
 }
 
 }  // namespace ice

@@ -44,6 +44,26 @@ __global__ void k_sort_done(DomainDesc *d, uint64_t pt_off, Counters *c, int cle
     }
 }
 
+// The off-tile points of every domain as one item space: pre holds the
+// exclusive prefix of the domains' list lengths (pre[ndom] = total), or is
+// null for a single domain.  The length of the space, and the domain of item
+// j: the last d with pre[d] <= j (domains with empty lists are skipped).
+__device__ __forceinline__ uint32_t offtile_total(const uint32_t *pre, int ndom, const Counters *cnt0) {
+    return pre ? pre[ndom] : cnt0[0].n_esc;
+}
+__device__ __forceinline__ int offtile_domain(const uint32_t *pre, int ndom, uint32_t j) {
+    int d = 0;
+    if (pre) {
+        int hi = ndom - 1;
+        while (d < hi) {
+            const int mid = (d + hi + 1) >> 1;
+            if (pre[mid] <= j) d = mid;
+            else hi = mid - 1;
+        }
+    }
+    return d;
+}
+
 struct TileRef {
     uint16_t d, tx, ty, pad;
 };
