@@ -71,3 +71,31 @@ def test_autotune_keeps_state():
                                       by_point(b.download(d)[0]).view(np.uint32))
     a.close()
     b.close()
+
+
+def test_batch_pipelined_run_with_offtile_lists_matches_oracle():
+    """48 substeps (three run chunks) of the batch on the pipelined path with
+    the drift re-sort off: points travel about two cells, so several domains'
+    off-tile lists fill at once and are walked as one item space; each domain
+    matches the FP64 oracle."""
+    print("test_batch_pipelined_run_with_offtile_lists_matches_oracle", flush=True)
+    doms = _domains()
+    b = BatchSimulation(doms, options={"resort_drift": 0})
+    lists = []
+    for c in range(3):
+        b.run(16 * c, 16, graph=True)
+        lists.append([b.info(d).n_offtile for d in range(len(doms))])
+    print("off-tile lists after each chunk", lists, flush=True)
+    assert any(sum(1 for v in row if v > 0) >= 2 for row in lists)
+    for d, sc in enumerate(doms):
+        pts, forces, err, dis = b.download(d)
+        o = oracle.Oracle.from_scene(sc, 64)
+        o.run(0, 48)
+        rep = compare_points(pts, o.points(), fields=[
+            ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
+            ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+            ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
+        ], max_bad_frac=0.01, max_flag_mismatch=0.01)
+        print("domain", d, "points", pts.shape[1], rep.get("Fe", {}).get("max"), flush=True)
+        assert err == o.error and dis == o.disabled
+    b.close()
