@@ -90,6 +90,7 @@ struct IceHandle {
     int ntiles = 0;
     bool tables_dirty = true;
     // sort scratch, sized for the largest domain
+    bool s_count_clean = false;   // every cell counter zero (k_sort_cells clears what a sort counted)
     uint32_t *s_count = nullptr, *s_key = nullptr, *s_slot = nullptr, *s_dflag = nullptr, *s_drank = nullptr,
              *s_order = nullptr, *s_bsum = nullptr;
     size_t s_bsum_len = 0;
@@ -416,7 +417,9 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
     const uint32_t NC = (uint32_t)D.N;
     uint32_t *cs = h->cs + D.cs_base;
     int rc;
-    if ((rc = launch_fill_u32(h, h->s_count, (size_t)NC + 1, 0u))) return rc;
+    // a sort that did not finish may have left counts anywhere: clear them all
+    if (!h->s_count_clean && (rc = launch_fill_u32(h, h->s_count, h->max_nodes() + 1, 0u))) return rc;
+    h->s_count_clean = false;
     if (n) {
         hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->dlines(d), n, D.gx, D.gy,
                            NC, h->s_count, h->s_key, h->s_slot, h->s_dflag, h->dcnt(d));
@@ -429,8 +432,10 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
                            h->s_slot, cs, h->s_drank, h->s_order);
         LAUNCHCHK(h, "k_sort_scatter");
     }
-    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(NC, 256)), dim3(256), 0, h->stream, NC, cs, h->s_order);
+    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(NC, 256)), dim3(256), 0, h->stream, NC, cs, h->s_order,
+                       h->s_count);
     LAUNCHCHK(h, "k_sort_cells");
+    h->s_count_clean = true;
     uint32_t n_active = n;   // only a sort that drops the disabled points needs the count on the host
     if (drop) {
         HIPCHK(h, hipMemcpyAsync(&n_active, cs + NC, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));

@@ -149,11 +149,14 @@ __global__ __launch_bounds__(256) void k_sort_scatter(uint32_t n, uint32_t NC, c
     order[pos] = i;
 }
 
-// Ascending input index inside each cell = the stable order.
+// Ascending input index inside each cell = the stable order.  Also clears
+// the cell counters the key pass filled (the scan has consumed them), so the
+// next sort starts from zeros without a fill.
 __global__ __launch_bounds__(256) void k_sort_cells(uint32_t NC, const uint32_t *__restrict__ cs,
-                                                    uint32_t *__restrict__ order) {
+                                                    uint32_t *__restrict__ order, uint32_t *__restrict__ count) {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= NC) return;
+    count[k] = 0;
     const uint32_t b = cs[k], e = cs[k + 1];
     for (uint32_t p = b + 1; p < e; p++) {
         const uint32_t v = order[p];
