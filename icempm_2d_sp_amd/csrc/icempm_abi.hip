@@ -393,17 +393,29 @@ int upload_tables(IceHandle *h) {
     return ICE_OK;
 }
 
-int scan_u32(IceHandle *h, const uint32_t *in, size_t n, uint32_t *out) {
-    const uint32_t nb = (uint32_t)std::max<size_t>(1, (n + kScanTile - 1) / kScanTile);
-    if ((size_t)nb + 1 > h->s_bsum_len) {
-        h->err = "scan scratch too small";
-        return ICE_E_STATE;
+// Two exclusive scans in one three-kernel pass (in_k[0..n_k) -> out_k[0..n_k],
+// out_k[n_k] = total), each with its own half of the block-sum scratch.
+int scan2_u32(IceHandle *h, const uint32_t *in0, size_t n0, uint32_t *out0, const uint32_t *in1, size_t n1,
+              uint32_t *out1) {
+    ScanJobs J;
+    const uint32_t *in[2] = {in0, in1};
+    const size_t n[2] = {n0, n1};
+    uint32_t *out[2] = {out0, out1};
+    uint32_t gx = 1;
+    for (int k = 0; k < 2; k++) {
+        const uint32_t nb = (uint32_t)std::max<size_t>(1, (n[k] + kScanTile - 1) / kScanTile);
+        if ((size_t)nb + 1 > h->s_bsum_len) {
+            h->err = "scan scratch too small";
+            return ICE_E_STATE;
+        }
+        J.j[k] = ScanJob{in[k], n[k], h->s_bsum + (size_t)k * h->s_bsum_len, nb, out[k]};
+        gx = std::max(gx, nb);
     }
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, h->stream, in, n, h->s_bsum);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(gx, 2), dim3(kScanThreads), 0, h->stream, J);
     LAUNCHCHK(h, "k_scan_reduce");
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, h->stream, h->s_bsum, nb);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1, 2), dim3(kScanThreads), 0, h->stream, J);
     LAUNCHCHK(h, "k_scan_blocks");
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, h->stream, in, n, h->s_bsum, nb, out);
+    hipLaunchKernelGGL(k_scan_apply, dim3(gx, 2), dim3(kScanThreads), 0, h->stream, J);
     LAUNCHCHK(h, "k_scan_apply");
     return ICE_OK;
 }
@@ -425,8 +437,7 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
                            NC, h->s_count, h->s_key, h->s_slot, h->s_dflag, h->dcnt(d));
         LAUNCHCHK(h, "k_sort_keys");
     }
-    if ((rc = scan_u32(h, h->s_count, NC, cs))) return rc;
-    if ((rc = scan_u32(h, h->s_dflag, n, h->s_drank))) return rc;
+    if ((rc = scan2_u32(h, h->s_count, NC, cs, h->s_dflag, n, h->s_drank))) return rc;
     if (n) {
         hipLaunchKernelGGL(k_sort_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, NC, h->s_key,
                            h->s_slot, cs, h->s_drank, h->s_order);
@@ -864,7 +875,7 @@ int alloc_points(IceHandle *h) {
     HIPCHK(h, hipMalloc(&h->s_drank, sizeof(uint32_t) * (mc + 1)));
     HIPCHK(h, hipMalloc(&h->s_order, sizeof(uint32_t) * mc));
     h->s_bsum_len = std::max<size_t>(mc, h->max_nodes()) / kScanTile + 8;
-    HIPCHK(h, hipMalloc(&h->s_bsum, sizeof(uint32_t) * h->s_bsum_len));
+    HIPCHK(h, hipMalloc(&h->s_bsum, sizeof(uint32_t) * 2 * h->s_bsum_len));   // one half per scan of a pass
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->tables_dirty = true;
     return ICE_OK;

@@ -42,9 +42,27 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
     return pre + x - v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ in, size_t n,
-                                                              uint32_t *__restrict__ bsum) {
+// One exclusive scan of a three-kernel pass: in[0..n) -> out[0..n], out[n] =
+// total; bsum holds nb + 1 block sums.  A pass runs one or two scans (grid
+// row blockIdx.y picks the job; workgroups past a job's blocks leave at once).
+struct ScanJob {
+    const uint32_t *in;
+    size_t n;
+    uint32_t *bsum;
+    uint32_t nb;
+    uint32_t *out;
+};
+struct ScanJobs {
+    ScanJob j[2];
+};
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(ScanJobs J) {
     __shared__ uint32_t s_w[kScanThreads / 64];
+    const ScanJob &jb = J.j[blockIdx.y];
+    if (blockIdx.x >= jb.nb) return;   // whole workgroup
+    const uint32_t *__restrict__ in = jb.in;
+    const size_t n = jb.n;
+    uint32_t *__restrict__ bsum = jb.bsum;
     const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     uint32_t s = 0;
     for (int k = 0; k < kScanItems; k++)
@@ -54,9 +72,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// One workgroup: exclusive scan of bsum[0..nb) in place, bsum[nb] = total.
-__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint32_t *__restrict__ bsum, uint32_t nb) {
+// One workgroup per job: exclusive scan of bsum[0..nb) in place, bsum[nb] = total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(ScanJobs J) {
     __shared__ uint32_t s_w[kScanThreads / 64];
+    uint32_t *__restrict__ bsum = J.j[blockIdx.y].bsum;
+    const uint32_t nb = J.j[blockIdx.y].nb;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nb; base += kScanTile) {
         uint32_t v[kScanItems];
@@ -79,10 +99,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(uint32_t *__restri
     if (threadIdx.x == 0) bsum[nb] = carry;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *__restrict__ in, size_t n,
-                                                             const uint32_t *__restrict__ bsum, uint32_t nb,
-                                                             uint32_t *__restrict__ out) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanJobs J) {
     __shared__ uint32_t s_w[kScanThreads / 64];
+    const ScanJob &jb = J.j[blockIdx.y];
+    if (blockIdx.x >= jb.nb) return;   // whole workgroup
+    const uint32_t *__restrict__ in = jb.in;
+    const size_t n = jb.n;
+    const uint32_t *__restrict__ bsum = jb.bsum;
+    const uint32_t nb = jb.nb;
+    uint32_t *__restrict__ out = jb.out;
     const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
