@@ -99,3 +99,30 @@ def test_batch_pipelined_run_with_offtile_lists_matches_oracle():
         print("domain", d, "points", pts.shape[1], rep.get("Fe", {}).get("max"), flush=True)
         assert err == o.error and dis == o.disabled
     b.close()
+
+
+def test_batch_det_with_offtile_lists_equals_single_domains_bitwise():
+    """The deterministic (unfused) path over three run chunks with the drift
+    re-sort off, so several domains' off-tile lists fill and the off-tile
+    kernels walk them as one item space: each domain of the batch stays bit
+    for bit equal to the same domain run alone."""
+    print("test_batch_det_with_offtile_lists_equals_single_domains_bitwise", flush=True)
+    doms = _domains()
+    flags = doms[0].params.flags | abi.FLAG_DETERMINISTIC
+    b = BatchSimulation(doms, flags=flags, options={"resort_drift": 0})
+    lists = []
+    for c in range(3):
+        b.run(16 * c, 16, graph=True)
+        lists.append([b.info(d).n_offtile for d in range(len(doms))])
+    print("off-tile lists after each chunk", lists, flush=True)
+    assert any(sum(1 for v in row if v > 0) >= 2 for row in lists)
+    for d, sc in enumerate(doms):
+        s = Simulation(sc, flags=flags, options={"resort_drift": 0})
+        s.run(0, 48, graph=True)
+        ps, fs, es, ds = s.download()
+        pb, fb, eb, db = b.download(d)
+        np.testing.assert_array_equal(by_point(pb).view(np.uint32), by_point(ps).view(np.uint32))
+        np.testing.assert_array_equal(fb.view(np.uint32), fs.view(np.uint32))
+        assert (eb, db) == (es, ds)
+        s.close()
+    b.close()
