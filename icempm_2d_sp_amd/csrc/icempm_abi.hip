@@ -95,7 +95,10 @@ struct IceHandle {
              *s_order = nullptr, *s_bsum = nullptr;
     size_t s_bsum_len = 0;
     // tiles
-    int tile_w = 16, tile_h = 32, band = 4;
+    // band 1: tiles walk down each column (y fastest), the order of the point
+    // and cell layout (x * GridY + y), so neighbouring workgroups read
+    // neighbouring memory (1-1.3 % faster than bands of 4 columns)
+    int tile_w = 16, tile_h = 32, band = 1;
     FixScale fix{};
     float max_thickness = 1.f;
     // phase state

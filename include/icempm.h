@@ -213,7 +213,7 @@ int ice_get_timings(IceHandle *h, double out[8]);
  * are tpb_P2G/Upd/G2P, parameters_sim.cpp:111-113). */
 int ice_set_tile(IceHandle *h, int tile_w, int tile_h);
 /* Tuning switches of the sorted path (no reference counterpart): "band"
- * (1..4096, default 4) is the number of tile columns per band of the
+ * (1..4096, default 1) is the number of tile columns per band of the
  * workgroup-to-tile order; "pipelined" (0 or 1, default 1) selects the
  * pipelined substep of ice_substep / ice_run; "persistent" (0 or 1, default
  * 0) runs its tile passes (without the surrogate) as persistent launches, as
