@@ -377,6 +377,20 @@ int upload_tables(IceHandle *h) {
                 for (int k = 0; k < wb; k++) tiles.push_back(TileRef{(uint16_t)d, (uint16_t)(b0 + k), (uint16_t)ty, 0});
         }
     }
+    if (nd > 1) {
+        // Workgroup b takes entry xcd_remap(b): one contiguous run of the table
+        // per XCD, which keeps shared halos in one L2 on a single domain.  Over
+        // uneven domains with empty tiles those runs carry unequal work, so a
+        // batch stores the table permuted to give workgroup b tile b (plain
+        // order, every XCD a share of every domain): 2 % faster on 64 domains.
+        const int nt = (int)tiles.size(), per = nt / 8, rem = nt % 8;
+        std::vector<TileRef> perm(tiles.size());
+        for (int b = 0; b < nt; b++) {
+            const int xcd = b % 8, j = b / 8;   // the device's xcd_remap
+            perm[(size_t)(xcd * per + std::min(xcd, rem) + j)] = tiles[(size_t)b];
+        }
+        tiles.swap(perm);
+    }
     if (tiles.size() > h->tiles_cap) {
         if (h->d_tiles) (void)hipFree(h->d_tiles);
         h->d_tiles = nullptr;
