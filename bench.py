@@ -7,20 +7,31 @@ Gaussian-random-field wind stress, lognormal per-point cohesion, dt = 1e-4 s.
 Inputs are resident in HBM when the timed region starts.  A "step" is one
 substep (P2G, node update, G2P with SVD and return map) over all points.
 
-The timed region replays K substeps from HIP graphs (16 substeps per graph,
-as ice_run does), bracketed by stream synchronisation; `value` = active
-points * K / wall time.  A second, shorter pass of the same substeps runs
-eagerly with HIP events around each kernel on the library's stream; it gives
-the dominant kernel's average launch time, from which `roofline` prices the
-kernel's algorithmic bytes per launch against MI355X's 8 TB/s.  The
-algorithmic bytes are those of the two-pass substep (P2G reads 16 point
-slots and writes 3 node floats; the node update + G2P reads 11 point slots
-with cohesion, writes 15, and reads 21 B per node: 168 B per point + 33 B per
-node); a launch of the pipelined tile pass is one whole substep.  The bytes
-the pipelined design itself must move (`own_design`) and the HBM bytes the
-counters measured (`traffic`) are reported beside them.
-`cpu_baseline` times the FP64 oracle on the host cores over a bounded sample
-of the same workload.
+Warm-up runs W eager substeps and then one 16-substep HIP-graph chunk, so the
+graph is captured and instantiated before timing.  The timed region replays K
+substeps from that graph (16 per replay; a remainder runs eagerly), bracketed
+by stream synchronisation; `value` = active points * K / wall time.  A second,
+shorter pass of the same substeps runs eagerly with HIP events around each
+kernel on the library's stream; it gives the dominant kernel's average launch
+time, from which `roofline` prices the kernel's algorithmic bytes per launch
+against MI355X's 8 TB/s.
+
+The dominant kernel, k_tile_pipeline, is a G2P2G pass: one launch runs the
+node update and G2P of substep n and the P2G of substep n + 1, whose inputs
+come from registers.  Its algorithmic bytes per launch are therefore those of
+the fused design: per point the G2P reads (utility, cell, pos x2, Fe x4,
+Jp_inv, thickness, cohesion: 44 B) and writes (utility, cell, pos x2, vel x2,
+Fe x4, Bp x4, Jp_inv: 60 B), 104 B; per node 33 B (P2G writes m, px, py; the
+update reads them, the status byte and the 2-float wind).  The two-pass
+substep's bytes (168 B per point + 33 B per node) priced on the same launch,
+the bytes the pipelined design itself must move (`own_design`) and the HBM
+bytes the counters measured (`traffic`) are reported beside them.
+
+Two secondary legs follow unless --no-secondary: the 64 uneven domains
+batched against the same domains run one after another (`batched`), and the
+64 domains in compression with the learned plastic projection against the
+plain forms (`surrogate`).  `cpu_baseline` times the FP64 oracle on the host
+cores over a bounded sample of the headline workload.
 
 Prints ONE JSON line.
 """
@@ -48,7 +59,10 @@ def algorithmic_bytes(n_points: int, n_nodes: int, cohesion: bool, complete_frac
     and writes utility, cell, pos x2, vel x2, Fe x4, Bp x4, Jp_inv (15 slots);
     per node it reads m, px, py, the status byte and the 2-float wind stress.
 
-    Pipelined (one tile pass per substep): per point the same 10 (+1) slots
+    Fused (G2P2G, one tile pass per substep, the P2G's point inputs taken
+    from registers): the G2P's point bytes plus 33 B per node.
+
+    Pipelined (the bytes that design itself moves): per point the same 10 (+1) slots
     read; position and Fe (6 slots) written every substep, velocity and Bp
     (6 slots) on the substeps that complete a run chunk (complete_frac of
     them), P and Q on recording substeps (rec_frac); Jp_inv, the utility and
@@ -58,7 +72,9 @@ def algorithmic_bytes(n_points: int, n_nodes: int, cohesion: bool, complete_frac
     p2g = 64 * n_points + 12 * n_nodes
     g2p = (40 + (4 if cohesion else 0) + 60) * n_points + 21 * n_nodes
     pipe = (40 + (4 if cohesion else 0) + 24 + 24 * complete_frac + 8 * rec_frac) * n_points + 37 * n_nodes
-    return {"p2g": p2g, "g2p": g2p, "substep": p2g + g2p, "pipelined": pipe}
+    # G2P2G: the G2P's point reads and writes, the P2G's and the update's node bytes
+    fused = (40 + (4 if cohesion else 0) + 60) * n_points + 33 * n_nodes
+    return {"p2g": p2g, "g2p": g2p, "substep": p2g + g2p, "pipelined": pipe, "fused": fused}
 
 
 def measured_traffic(kernel: str, tile: str, grid: str):
@@ -125,6 +141,9 @@ def main() -> int:
     ap.add_argument("--surrogate-tol", type=float, default=1e-4, help="yield-violation tolerance of the fallback")
     ap.add_argument("--copy-only", type=float, default=0.0,
                     help="only time a plain copy of this many GB (HBM reference / counter calibration)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the batched and surrogate legs of the default workload")
+    ap.add_argument("--secondary-steps", type=int, default=32, help="timed substeps of each secondary leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -170,14 +189,16 @@ def main() -> int:
         opts["resort_drift"] = args.resort_drift
     sim = Simulation(scene, flags=flags, tile=(tw, th), options=opts)
     n_active = sim.info().n_points
-    print(f"[bench] uploaded and sorted; warmup {args.warmup}", file=sys.stderr, flush=True)
+    print(f"[bench] uploaded and sorted; warmup {args.warmup} + one graph chunk", file=sys.stderr, flush=True)
     sim.run(0, args.warmup, graph=False)
+    sim.run(args.warmup, 16, graph=True)     # graph capture and instantiation, outside the timer
     sim.sync()
+    first = args.warmup + 16
 
     # timed region: graph replays of whole substeps
     sorts0 = sim.info().n_sorts
     t0 = time.perf_counter()
-    sim.run(args.warmup, args.steps, graph=True)
+    sim.run(first, args.steps, graph=True)
     sim.sync()
     wall = time.perf_counter() - t0
     sorts_timed = sim.info().n_sorts - sorts0
@@ -185,7 +206,7 @@ def main() -> int:
     print(f"[bench] timed {args.steps} substeps in {wall:.3f}s; err {err:#x}", file=sys.stderr, flush=True)
 
     # per-kernel times: the next substeps, eagerly, with HIP events
-    ev_first, ev_steps = args.warmup + args.steps, args.event_steps
+    ev_first, ev_steps = first + args.steps, args.event_steps
     sim.reset_timings()
     sim.enable_timings(True)
     sim.sync()
@@ -208,32 +229,42 @@ def main() -> int:
     B = algorithmic_bytes(n_active, nodes, True, complete_frac=len(chunks) / ev_steps, rec_frac=n_rec / ev_steps)
     if pipelined:
         # one launch is one whole substep (the node update and G2P of substep n,
-        # the P2G of n + 1): its algorithmic bytes are the substep's two-pass
-        # figure, 168 B per point + 33 B per node; the bytes this design itself
-        # has to move are reported beside it (own_design)
-        dominant, ms_dom, kname = "substep", ms_g2p, "k_tile_pipeline"
+        # the P2G of n + 1 from registers): priced at the fused design's bytes,
+        # 104 B per point + 33 B per node; the two-pass figure and the bytes this
+        # design itself has to move are reported beside it
+        dominant, ms_dom, kname = "fused", ms_g2p, "k_tile_pipeline"
     elif ms_g2p >= ms_p2g:
         dominant, ms_dom, kname = "g2p", ms_g2p, "k_g2p_tiles" if not args.naive else "k_g2p_naive"
     else:
         dominant, ms_dom, kname = "p2g", ms_p2g, "k_p2g_tiles" if not args.naive else "k_p2g_naive"
     achieved = B[dominant] / (ms_dom * 1e-3) / 1e9
     value = n_active * args.steps / wall
-    # the two-pass byte count per wall-clock substep (every kernel and gap of
-    # the graph replays included)
-    two_pass = B["substep"] / (ms_step * 1e-3) / 1e9
-    own = None
+    # the dominant kernel's bytes per wall-clock substep (every kernel and gap
+    # of the graph replays included) and per event-timed substep
+    B_sub = B["fused"] if pipelined else B["substep"]
+    wall_gbs = B_sub / (ms_step * 1e-3) / 1e9
+    ev_gbs = B_sub / (ms_sub_ev * 1e-3) / 1e9
+    own = two_pass_launch = None
     if pipelined:
         own_gbs = B["pipelined"] / (ms_dom * 1e-3) / 1e9
         own = {"bytes": B["pipelined"], "GBs": own_gbs, "frac": own_gbs / HBM_PEAK_GBS}
+        tp = B["substep"] / (ms_dom * 1e-3) / 1e9
+        two_pass_launch = {"bytes": B["substep"], "GBs": tp, "frac": tp / HBM_PEAK_GBS,
+                           "note": "the two-pass substep's 168 B/point + 33 B/node priced on the same launch"}
+    sim.close()
+
+    secondary = {}
+    if not args.no_secondary:
+        secondary["batched"] = measure_batch(args.secondary_steps, autotune=False)
+        secondary["surrogate"] = measure_surrogate(args.secondary_steps, args.surrogate_tol)
 
     cpu = None
     if args.cpu_seconds > 0:
-        sim.close()
         print("[bench] CPU baseline (FP64 oracle)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(scene, args.cpu_seconds, args.cpu_threads)
 
     out = {
-        "metric": "particle-updates/s per substep at 32M particles (plain FP32 substep)",
+        "metric": "particle-updates/s per substep at 32M particles (plain FP32 substep; surrogate leg beside)",
         "value": value,
         "unit": "particle-updates/s",
         "n_gpus": 1,
@@ -260,209 +291,219 @@ def main() -> int:
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(kname, f"{tw}x{th}", f"{gx}x{gy}"),
             "bytes_per_launch": B[dominant], "ms_per_launch": ms_dom,
-            "substep": {"bytes": B["substep"], "ms_events": ms_sub_ev,
-                        "achieved_GBs": B["substep"] / (ms_sub_ev * 1e-3) / 1e9,
-                        "frac": B["substep"] / (ms_sub_ev * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "substep_events": {"bytes": B_sub, "ms": ms_sub_ev, "GBs": ev_gbs, "frac": ev_gbs / HBM_PEAK_GBS},
+            "substep_wall_clock": {"bytes": B_sub, "ms": ms_step, "GBs": wall_gbs, "frac": wall_gbs / HBM_PEAK_GBS},
             "p2g_ms": ms_p2g, "g2p_ms": ms_g2p, "event_substeps": ev_steps,
+            "two_pass_bytes_same_launch": two_pass_launch,
             "own_design": own,
-            "wall_clock_substep": {"GBs": two_pass, "frac": two_pass / HBM_PEAK_GBS},
         },
         "cpu_baseline": cpu,
     }
+    out.update(secondary)
     print(json.dumps(out), flush=True)
     return 0
 
 
-def bench_batch(args) -> int:
-    """batch64_uneven: 64 independent domains (sides 128-1024 cells, 25-95 % ice) run
-    (a) as one batched handle (one launch per phase, HIP-graph replays, tile
-    autotuned) and (b) one domain after another with the single-domain path,
-    the reference's one-partition-at-a-time shape.  value = (a)."""
+def _timed_run(sim, steps: int) -> float:
+    """One graph chunk of warm-up (capture, instantiation), then `steps`
+    substeps of graph replays, timed."""
+    sim.run(0, 16, graph=True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(16, steps, graph=True)
+    sim.sync()
+    return time.perf_counter() - t0
+
+
+def measure_batch(steps: int, autotune: bool = True, persistent: bool = False, resort_drift=None) -> dict:
+    """batch64_uneven: 64 independent domains (sides 128-1024 cells, 25-95 % ice)
+    run (a) as one batched handle (one launch per phase, HIP-graph replays) and
+    (b) one domain after another with the single-domain path, the reference's
+    one-partition-at-a-time shape, each over the same `steps` substeps."""
     from icempm_2d_sp_amd import scenes
     from icempm_2d_sp_amd.gpu import BatchSimulation, Simulation
     t0 = time.time()
     doms = scenes.batch64_uneven()
     print(f"[bench] {len(doms)} domains, {sum(d.n for d in doms)} points, generated in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
-    steps = max(16, (args.steps // 16) * 16)
-    opts = {"persistent": int(args.persistent)}
-    if args.resort_drift is not None:
-        opts["resort_drift"] = args.resort_drift
+    steps = max(16, (steps // 16) * 16)
+    opts = {"persistent": int(persistent)}
+    if resort_drift is not None:
+        opts["resort_drift"] = resort_drift
     b = BatchSimulation(doms, options=opts)
-    tile, tune_ms = b.autotune(2)
+    tile, tune_ms = b.autotune(2) if autotune else ((16, 32), None)
     n_active = b.n_points()
-    b.run(0, 16, graph=True)
-    b.sync()
     sorts0 = sum(b.info(d).n_sorts for d in range(len(doms)))
-    t0 = time.perf_counter()
-    b.run(16, steps, graph=True)
-    b.sync()
-    wall_b = time.perf_counter() - t0
+    wall_b = _timed_run(b, steps)
     sorts_b = sum(b.info(d).n_sorts for d in range(len(doms))) - sorts0
     b.close()
     print(f"[bench] batched: {steps} substeps in {wall_b:.3f}s (tile {tile})", file=sys.stderr, flush=True)
-    # one domain at a time, each for the same substeps
     wall_s = 0.0
     for sc in doms:
-        s = Simulation(sc)
-        s.run(0, 16, graph=True)
-        s.sync()
-        t0 = time.perf_counter()
-        s.run(16, steps, graph=True)
-        s.sync()
-        wall_s += time.perf_counter() - t0
-        s.close()
+        s1 = Simulation(sc)
+        wall_s += _timed_run(s1, steps)
+        s1.close()
     print(f"[bench] sequential: {wall_s:.3f}s", file=sys.stderr, flush=True)
-    out = {
-        "metric": "particle-updates/s per substep, 64 batched uneven domains (plain FP32 substep)",
-        "value": n_active * steps / wall_b,
-        "unit": "particle-updates/s",
-        "n_gpus": 1, "steps": steps, "warmup": 16, "ms_per_step": wall_b * 1000 / steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batch64_uneven: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, 4 points/cell, "
-                               "per-domain wind 0.05-0.3 Pa", "points": n_active, "tile": f"{tile[0]}x{tile[1]}",
-                   "sorts_in_timed_region": sorts_b},
+    return {
+        "workload": "batch64_uneven: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, 4 points/cell, "
+                    "per-domain wind 0.05-0.3 Pa, plain FP32 substep",
+        "points": n_active, "steps": steps, "tile": f"{tile[0]}x{tile[1]}", "sorts_in_timed_region": sorts_b,
+        "value": n_active * steps / wall_b, "unit": "particle-updates/s", "ms_per_step": wall_b * 1000 / steps,
         "sequential_single_domain": {"value": n_active * steps / wall_s, "ms_per_step": wall_s * 1000 / steps},
         "batched_over_sequential": wall_s / wall_b,
+    }
+
+
+def bench_batch(args) -> int:
+    """batch64_uneven (measure_batch), tile autotuned; value = the batched rate."""
+    r = measure_batch(args.steps, autotune=True, persistent=args.persistent, resort_drift=args.resort_drift)
+    out = {
+        "metric": "particle-updates/s per substep, 64 batched uneven domains (plain FP32 substep)",
+        "value": r["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": r["steps"], "warmup": 16,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": r["workload"], "points": r["points"], "tile": r["tile"],
+                   "sorts_in_timed_region": r["sorts_in_timed_region"]},
+        "sequential_single_domain": r["sequential_single_domain"],
+        "batched_over_sequential": r["batched_over_sequential"],
     }
     print(json.dumps(out), flush=True)
     return 0
 
 
-def bench_surrogate(args) -> int:
+def train_surrogate(sim, pairs: int = 1 << 20):
+    """Train the learned plastic projection on the GPU on `sim`'s handle:
+    `pairs` training pairs from the exact projection, then 20,000 Adam steps
+    of 8192 pairs.  Returns (loss curve, seconds)."""
+    import numpy as np
+    t0 = time.perf_counter()
+    sim.mlp_generate(pairs, seed=3)
+    sim.mlp_init(seed=21)
+    loss = np.concatenate([sim.mlp_train(8000, batch=8192, lr=2e-3, seed=5),
+                           sim.mlp_train(6000, batch=8192, lr=5e-4, seed=6),
+                           sim.mlp_train(6000, batch=8192, lr=1e-4, seed=7)])
+    return loss, time.perf_counter() - t0
+
+
+def surrogate_accuracy(sim, tol: float, n: int = 1 << 16, seed: int = 99) -> dict:
+    """Stress error of the deployed network against the FP64 exact projection
+    on `n` held-out trial states (another seed, another handle's pairs)."""
+    import numpy as np
+    from icempm_2d_sp_amd import scenes
+    from icempm_2d_sp_amd.gpu import Simulation
+    aux = Simulation(scenes.floe_256x128())
+    x_test, _, _ = aux.mlp_generate(n, seed=seed, copy=True)
+    aux.close()
+    out = sim.mlp_eval(x_test, tol)
+    ps, qs, pe, qe, viol, acc = (out[:, k].astype(np.float64) for k in range(6))
+    err = np.hypot(ps - pe, qs - qe) / np.hypot(pe, qe)
+    accepted = acc > 0.5
+    err_dep = np.where(accepted, err, 0.0)   # fallback points take the exact map
+    return {"network_median": float(np.median(err)), "network_p99": float(np.percentile(err, 99)),
+            "network_max": float(err.max()), "accepted_fraction_heldout": float(accepted.mean()),
+            "deployed_p99": float(np.percentile(err_dep, 99)), "deployed_max": float(err_dep.max()),
+            "heldout_states": int(n)}
+
+
+def measure_surrogate(steps: int, tol: float, naive_leg: bool = False) -> dict:
     """batch64_compressed: 64 uneven domains in uniaxial compression at 1/s
     against the fixed wall at x_max under 1 Pa wind (most points project
-    plastically every substep).  Times
-    (a) plain batched substeps (exact return map), (b) batched substeps with
-    the learned plastic projection and the exact fallback at
-    --surrogate-tol, and (c) the plain single-domain runs one after another.
-    The network is trained on the GPU first (not timed); its stress error
-    against the FP64 exact projection is measured on held-out trial states.
-    value = (b)."""
+    plastically every substep).  Times (a) plain batched substeps (exact
+    return map), (b) batched substeps with the learned plastic projection and
+    the exact fallback at `tol`, (c) the plain single-domain runs one after
+    another, and with naive_leg (d) the reference's launch structure one
+    domain after another.  The network is trained on the GPU first (not
+    timed)."""
     import numpy as np
     from icempm_2d_sp_amd import abi, scenes
     from icempm_2d_sp_amd.gpu import BatchSimulation, Simulation
     doms = scenes.batch64_compressed()
-    steps = max(16, (args.steps // 16) * 16)
-    tol = args.surrogate_tol
+    steps = max(16, (steps // 16) * 16)
     print(f"[bench] {len(doms)} domains, {sum(d.n for d in doms)} points; {steps} substeps", file=sys.stderr,
           flush=True)
-
-    def timed(sim):
-        sim.run(0, 16, graph=True)
-        sim.sync()
-        t0 = time.perf_counter()
-        sim.run(16, steps, graph=True)
-        sim.sync()
-        return time.perf_counter() - t0
-
     plain = BatchSimulation(doms)
     n_active = plain.n_points()
-    wall_plain = timed(plain)
+    wall_plain = _timed_run(plain, steps)
+    pa, _, _, _ = plain.download(0)
+    plain.close()
     print(f"[bench] plain batched {wall_plain:.3f}s", file=sys.stderr, flush=True)
 
     surr = BatchSimulation(doms)
-    t0 = time.perf_counter()
-    surr.mlp_generate(1 << 20, seed=3)
-    surr.mlp_init(seed=21)
-    loss = np.concatenate([surr.mlp_train(8000, batch=8192, lr=2e-3, seed=5),
-                           surr.mlp_train(6000, batch=8192, lr=5e-4, seed=6),
-                           surr.mlp_train(6000, batch=8192, lr=1e-4, seed=7)])
-    t_train = time.perf_counter() - t0
-    trained, trained_norm = surr.mlp_params(), surr.mlp_norm()
+    loss, t_train = train_surrogate(surr)
     print(f"[bench] trained in {t_train:.2f}s, loss {loss[0]:.3g} -> {loss[-100:].mean():.3g}", file=sys.stderr,
           flush=True)
-    # a looser tolerance first (reported beside), then the requested one (value)
-    loose = 1e-2
-    surr.mlp_deploy(True, loose)
-    wall_loose = timed(surr)
-    acc_loose, fb_loose = surr.mlp_counts()
-    surr.close()
-    surr = BatchSimulation(doms)
-    surr.mlp_set_norm(trained_norm)
-    surr.mlp_set_params(trained)
     surr.mlp_deploy(True, tol)
-    wall_surr = timed(surr)
+    wall_surr = _timed_run(surr, steps)
     n_acc, n_fb = surr.mlp_counts()
-    print(f"[bench] surrogate batched {wall_surr:.3f}s, projections {n_acc}, fallbacks {n_fb}; at tol {loose:g}: "
-          f"{wall_loose:.3f}s, projections {acc_loose}, fallbacks {fb_loose}", file=sys.stderr, flush=True)
-    # accuracy on held-out trial states (another seed, another handle's pairs)
-    aux = Simulation(doms[0])
-    x_test, _, _ = aux.mlp_generate(1 << 16, seed=99, copy=True)
-    aux.close()
-    out = surr.mlp_eval(x_test, tol)
-    ps, qs, pe, qe, viol, acc = (out[:, k].astype(np.float64) for k in range(6))
-    err = np.hypot(ps - pe, qs - qe) / np.hypot(pe, qe)
-    accepted = acc > 0.5
-    err_deployed = np.where(accepted, err, 0.0)   # fallback points take the exact map
-    # final state of the first domain: surrogate run vs plain run
-    pa, _, _, _ = plain.download(0)
-    pb, _, _, _ = surr.download(0)
+    acc = surrogate_accuracy(surr, tol)
+    pb, _, err_surr, _ = surr.download(0)
+    surr.close()
+    print(f"[bench] surrogate batched {wall_surr:.3f}s, projections {n_acc}, fallbacks {n_fb}", file=sys.stderr,
+          flush=True)
     ka = np.argsort(pa[abi.IDX_POINT].view(np.uint32), kind="stable")
     kb = np.argsort(pb[abi.IDX_POINT].view(np.uint32), kind="stable")
-    fe_a = pa[abi.IDX_FE00:abi.IDX_FE00 + 4][:, ka].astype(np.float64)
-    fe_b = pb[abi.IDX_FE00:abi.IDX_FE00 + 4][:, kb].astype(np.float64)
-    dfe = float(np.abs(fe_a - fe_b).max())
-    plain.close()
-    surr.close()
-    # plain single-domain runs, one after another (the reference's launch shape)
+    dfe = float(np.abs(pa[abi.IDX_FE00:abi.IDX_FE00 + 4][:, ka].astype(np.float64) -
+                       pb[abi.IDX_FE00:abi.IDX_FE00 + 4][:, kb]).max())
     wall_seq = 0.0
     for sc in doms:
         s1 = Simulation(sc)
-        wall_seq += timed(s1)
+        wall_seq += _timed_run(s1, steps)
         s1.close()
     print(f"[bench] plain sequential {wall_seq:.3f}s", file=sys.stderr, flush=True)
-    # the reference's own launch structure (unsorted points, global atomics,
-    # ICE_FLAG_NAIVE), one domain after another, over 64 substeps
-    naive_steps = 64
-    wall_naive = 0.0
-    for sc in doms:
-        s1 = Simulation(sc, flags=sc.params.flags | abi.FLAG_NAIVE)
-        s1.run(0, 16, graph=True)
-        s1.sync()
-        t0 = time.perf_counter()
-        s1.run(16, naive_steps, graph=True)
-        s1.sync()
-        wall_naive += time.perf_counter() - t0
-        s1.close()
-    ms_naive = wall_naive * 1000 / naive_steps
-    print(f"[bench] plain sequential, reference launch structure: {ms_naive:.3f} ms per substep", file=sys.stderr,
-          flush=True)
-    plastic = (n_acc + n_fb) / (n_active * steps)
-    outj = {
-        "metric": "particle-updates/s per substep, 64 batched uneven domains, surrogate + batched vs plain",
-        "value": n_active * steps / wall_surr,
-        "unit": "particle-updates/s",
-        "n_gpus": 1, "steps": steps, "warmup": 16, "ms_per_step": wall_surr * 1000 / steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 state, bf16 MLP",
-        "data": "synthetic",
-        "config": {"workload": "batch64_compressed: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, "
-                               "uniaxial compression at 1/s against the fixed wall at x_max, 1.0 Pa wind along +x, 4 points/cell",
-                   "points": n_active, "surrogate_tol": tol},
+    out = {
+        "workload": "batch64_compressed: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, uniaxial "
+                    "compression at 1/s against the fixed wall at x_max, 1.0 Pa wind along +x, 4 points/cell",
+        "points": n_active, "steps": steps, "surrogate_tol": tol, "dtype": "f32 state, bf16 MLP",
+        "value": n_active * steps / wall_surr, "unit": "particle-updates/s", "ms_per_step": wall_surr * 1000 / steps,
         "plain_batched": {"value": n_active * steps / wall_plain, "ms_per_step": wall_plain * 1000 / steps},
         "plain_sequential": {"value": n_active * steps / wall_seq, "ms_per_step": wall_seq * 1000 / steps},
-        "plain_sequential_reference_launch_structure": {
-            "value": n_active * 1000 / ms_naive, "ms_per_step": ms_naive, "substeps": naive_steps,
-            "note": "unsorted points, P2G with global atomics, separate update and G2P (ICE_FLAG_NAIVE)"},
         "surrogate_batched_over_plain_sequential": wall_seq / wall_surr,
-        "surrogate_batched_over_plain_sequential_reference_launch_structure":
-            ms_naive / (wall_surr * 1000 / steps),
         "surrogate_batched_over_plain_batched": wall_plain / wall_surr,
-        "ridge_capped_plastic_fraction_per_substep": plastic,
+        "ridge_capped_plastic_fraction_per_substep": (n_acc + n_fb) / (n_active * steps),
         "surrogate_accepted_fraction": n_acc / max(n_acc + n_fb, 1),
-        "at_tol_1e-2": {"value": n_active * steps / wall_loose, "ms_per_step": wall_loose * 1000 / steps,
-                        "accepted_fraction": acc_loose / max(acc_loose + fb_loose, 1),
-                        "over_plain_sequential": wall_seq / wall_loose, "over_plain_batched": wall_plain / wall_loose},
         "training": {"pairs": 1 << 20, "steps": int(loss.shape[0]), "seconds": t_train,
                      "loss_first": float(loss[0]), "loss_last": float(loss[-100:].mean())},
-        "stress_error_vs_fp64": {
-            "network_median": float(np.median(err)), "network_p99": float(np.percentile(err, 99)),
-            "network_max": float(err.max()),
-            "accepted_fraction_heldout": float(accepted.mean()),
-            "deployed_max": float(err_deployed.max()), "deployed_mean": float(err_deployed.mean())},
-        "final_Fe_max_abs_diff_first_domain": dfe,
+        "stress_error_vs_fp64": acc,
+        "final_Fe_max_abs_diff_first_domain": dfe, "error_word": err_surr,
     }
-    print(json.dumps(outj), flush=True)
+    if naive_leg:
+        # the reference's own launch structure (unsorted points, global atomics,
+        # ICE_FLAG_NAIVE, eager launches), one domain after another
+        naive_steps = 64
+        wall_naive = 0.0
+        for sc in doms:
+            s1 = Simulation(sc, flags=sc.params.flags | abi.FLAG_NAIVE)
+            s1.run(0, 16, graph=False)
+            s1.sync()
+            t0 = time.perf_counter()
+            s1.run(16, naive_steps, graph=False)
+            s1.sync()
+            wall_naive += time.perf_counter() - t0
+            s1.close()
+        ms_naive = wall_naive * 1000 / naive_steps
+        print(f"[bench] plain sequential, reference launch structure: {ms_naive:.3f} ms per substep",
+              file=sys.stderr, flush=True)
+        out["plain_sequential_reference_launch_structure"] = {
+            "value": n_active * 1000 / ms_naive, "ms_per_step": ms_naive, "substeps": naive_steps,
+            "note": "unsorted points, P2G with global atomics, separate update and G2P (ICE_FLAG_NAIVE)"}
+        out["surrogate_batched_over_plain_sequential_reference_launch_structure"] = ms_naive / (wall_surr * 1000 / steps)
+    return out
+
+
+def bench_surrogate(args) -> int:
+    """batch64_compressed (measure_surrogate, with the naive leg); value =
+    the surrogate batched rate."""
+    r = measure_surrogate(args.steps, args.surrogate_tol, naive_leg=True)
+    out = {
+        "metric": "particle-updates/s per substep, 64 batched uneven domains, surrogate + batched vs plain",
+        "value": r["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": r["steps"], "warmup": 16,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": r["dtype"], "data": "synthetic",
+        "config": {"workload": r["workload"], "points": r["points"], "surrogate_tol": r["surrogate_tol"]},
+    }
+    out.update({k: v for k, v in r.items() if k not in out and k not in ("workload", "points", "steps",
+                                                                         "surrogate_tol", "dtype")})
+    print(json.dumps(out), flush=True)
     return 0
 
 

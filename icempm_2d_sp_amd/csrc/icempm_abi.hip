@@ -329,8 +329,14 @@ GridLines grid_lines(const IceHandle *h) {
     return G;
 }
 
+// Destroy the captured graph.  A launch of it may still be in flight (ice_run
+// returns without waiting on paths that need no counters), so the stream is
+// drained first: an executable graph is never destroyed under its own launch.
 void drop_graph(IceHandle *h) {
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph_exec) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipGraphExecDestroy(h->graph_exec);
+    }
     h->graph_exec = nullptr;
     h->graph_chunk = 0;
 }
@@ -1316,7 +1322,8 @@ int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
             hipLaunchKernelGGL(k_clear_drift, dim3(1), dim3(256), 0, h->stream, h->cnt, h->ndom());
             LAUNCHCHK(h, "k_clear_drift");
         }
-        if (use_graph && chunk == kChunk && !h->timing) {
+        // the naive path is the reference's launch structure: always eager
+        if (use_graph && chunk == kChunk && !h->timing && !h->naive()) {
             if (!h->graph_exec || h->graph_chunk != chunk || h->graph_fr0 != h->fr_cur) {
                 if ((rc = build_graph(h, chunk))) return rc;
             }

@@ -414,8 +414,12 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                         FInv vf;
                         if (act) {
                             const int rx = s.cx - fx0, ry = s.cy - fy0;
-                            if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2) {
-                                err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
+                            // outside the frame cannot happen while the drift invariant
+                            // holds; a stencil outside the grid (an uploaded point at the
+                            // edge) is skipped, as by the other paths
+                            if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2 ||
+                                !stencil_inside(s.cx, s.cy, P)) {
+                                err |= ICE_ERR_POINT_LEFT_AREA;
                                 act = false;
                             } else {
                                 float wx[3], wy[3];

@@ -75,8 +75,10 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiles(DevParams P0, StepAr
         const float dpt = dp_threshold(P, L, i);
         if (u & (kDisabled | kEscaped)) continue;
         const int rx = s.cx - x0 + 2, ry = s.cy - y0 + 2;
-        if (rx < 1 || rx > RW - 2 || ry < 1 || ry > RH - 2) {
-            err |= ICE_ERR_POINT_LEFT_AREA;   // cannot happen while the drift invariant holds
+        // outside the frame cannot happen while the drift invariant holds; a
+        // stencil outside the grid (an uploaded point at the edge) is skipped
+        if (rx < 1 || rx > RW - 2 || ry < 1 || ry > RH - 2 || !stencil_inside(s.cx, s.cy, P)) {
+            err |= ICE_ERR_POINT_LEFT_AREA;
             continue;
         }
         float wx[3], wy[3];

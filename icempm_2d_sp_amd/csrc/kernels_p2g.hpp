@@ -22,13 +22,19 @@
 
 namespace ice {
 
+// One contribution in fixed point.  A NaN contribution (flagged by the
+// caller) is dropped: integer sums cannot carry it.
+__device__ __forceinline__ long long to_fix(float v, float scale) {
+    return __builtin_isnan(v) ? 0ll : (long long)(v * scale);
+}
+
 template <bool DET, int NN>
 __device__ __forceinline__ void lds_add3(typename std::conditional<DET, unsigned long long, float>::type *acc, int o,
                                          float dm, float dpx, float dpy, const FixScale &fs) {
     if constexpr (DET) {
-        atomicAdd(&acc[o], (unsigned long long)(long long)(dm * fs.m));
-        atomicAdd(&acc[NN + o], (unsigned long long)(long long)(dpx * fs.p));
-        atomicAdd(&acc[2 * NN + o], (unsigned long long)(long long)(dpy * fs.p));
+        atomicAdd(&acc[o], (unsigned long long)to_fix(dm, fs.m));
+        atomicAdd(&acc[NN + o], (unsigned long long)to_fix(dpx, fs.p));
+        atomicAdd(&acc[2 * NN + o], (unsigned long long)to_fix(dpy, fs.p));
     } else {
         __hip_atomic_fetch_add(&acc[o], dm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         __hip_atomic_fetch_add(&acc[NN + o], dpx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -97,9 +103,9 @@ __device__ __forceinline__ bool cell_stencil(const Lines &L, const DevParams &P,
                 if constexpr (DET) {   // the conversion scatter_point's contributions take
                     if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
                     if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
-                    am[a][bb] += (long long)(dm * fs.m);
-                    ax[a][bb] += (long long)(dpx * fs.p);
-                    ay[a][bb] += (long long)(dpy * fs.p);
+                    am[a][bb] += to_fix(dm, fs.m);
+                    ax[a][bb] += to_fix(dpx, fs.p);
+                    ay[a][bb] += to_fix(dpy, fs.p);
                 } else {
                     am[a][bb] += dm;
                     ax[a][bb] += dpx;

@@ -78,15 +78,16 @@ class SimParams:
 
 def sea_ice_params(gx: int, gy: int, *, dt: float = 1e-4, cohesion: float = 1e5, phi_deg: float = 30.0,
                    drag_cd: float = 0.005, ice_thickness: float = 1.0, frame_steps: int = 100,
-                   flags: int = 0) -> SimParams:
+                   flags: int = 0, cellsize: float = 1.0) -> SimParams:
     """Material and forcing of the MPM workloads: E = 1e9 Pa, nu = 0.3,
     rho = 916, Drucker-Prager phi with cohesion c (DP_threshold_p = -c/tan phi,
     the apex of q = (p - DP_threshold_p) tan phi used at kernels.cuh:422),
     quadratic water drag rho_w Cd / (rho_i h_ice), wind stress acceleration
-    1/(rho_i h_ice).  ParticleVolume is set by the scene generator."""
+    1/(rho_i h_ice), cell size h (m).  ParticleVolume is set by the scene
+    generator."""
     rho_i, rho_w = 916.0, 1030.0
     return SimParams(
-        GridXTotal=gx, GridYTotal=gy, cellsize=1.0, InitialTimeStep=dt,
+        GridXTotal=gx, GridYTotal=gy, cellsize=cellsize, InitialTimeStep=dt,
         AnimationFramePeriod=frame_steps * dt * (1 + 1e-9),
         IceDensity=rho_i, PoissonsRatio=0.3, YoungsModulus=1e9,
         IceCompressiveStrength=100e6, IceTensileStrength=3e6, IceShearStrength=2.5e6,

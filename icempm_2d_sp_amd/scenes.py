@@ -174,13 +174,15 @@ def icefield_4096x2048(seed: int = 2024, gx: int = 4096, gy: int = 2048, cohesio
 
 def dynamic_scene(seed: int = 7, gx: int = 48, gy: int = 40, *, speed: float = 300.0, strain: float = 4e-3,
                   crushed_frac: float = 0.5, with_cohesion: bool = False, with_current: bool = True,
-                  fill: float = 0.8) -> Scene:
+                  fill: float = 0.8, cellsize: float = 1.0, bp_scale: float = 5.0) -> Scene:
     """A small, strongly perturbed scene for parity tests: a swirling velocity
-    field of up to `speed` m/s (cells cross every ~10-100 substeps at dt =
-    1e-4), random elastic strain up to `strain`, a fraction of points already
-    crushed, random Jp_inv and thickness, a random current and wind.  Exercises
-    every branch of the update and return map, cell crossings, the disable
-    band and the wall force accumulator."""
+    field of up to `speed` cells/s (speed * cellsize m/s; cells cross every
+    ~10-100 substeps at dt = 1e-4 with the default speed), random elastic
+    strain up to `strain`, APIC matrices Bp of `bp_scale` 1/s, a fraction of
+    points already crushed, random Jp_inv and thickness, a random current and
+    wind, on a grid of cell size `cellsize` m.  Exercises every branch of the
+    update and return map, cell crossings, the disable band and the wall force
+    accumulator."""
     rng = np.random.default_rng(seed)
     x, y, n_gen = stratified_positions(gx, gy, rng)
     cx, cy = (gx - 1) / 2, (gy - 1) / 2
@@ -192,11 +194,12 @@ def dynamic_scene(seed: int = 7, gx: int = 48, gy: int = 40, *, speed: float = 3
     yi = (u32_line(soa, abi.IDX_CELL) >> 16).astype(np.float64) + soa[abi.IDX_POSX + 1]
     ang = np.arctan2(yi - cy, xi - cx)
     rad = np.hypot((xi - cx) / cx, (yi - cy) / cy)
-    soa[abi.IDX_VELX] = (-speed * rad * np.sin(ang) + 0.2 * speed * rng.standard_normal(n)).astype(np.float32)
-    soa[abi.IDX_VELX + 1] = (speed * rad * np.cos(ang) + 0.2 * speed * rng.standard_normal(n)).astype(np.float32)
+    vs = speed * cellsize
+    soa[abi.IDX_VELX] = (-vs * rad * np.sin(ang) + 0.2 * vs * rng.standard_normal(n)).astype(np.float32)
+    soa[abi.IDX_VELX + 1] = (vs * rad * np.cos(ang) + 0.2 * vs * rng.standard_normal(n)).astype(np.float32)
     F = np.eye(2)[None] + strain * rng.standard_normal((n, 2, 2))
     soa[abi.IDX_FE00:abi.IDX_FE00 + 4] = F.reshape(n, 4).T.astype(np.float32)
-    soa[abi.IDX_BP00:abi.IDX_BP00 + 4] = (5.0 * rng.standard_normal((4, n))).astype(np.float32)
+    soa[abi.IDX_BP00:abi.IDX_BP00 + 4] = (bp_scale * rng.standard_normal((4, n))).astype(np.float32)
     soa[abi.IDX_JP_INV] = rng.uniform(0.05, 1.3, n).astype(np.float32)
     soa[abi.IDX_THICKNESS] = rng.uniform(0.5, 2.0, n).astype(np.float32)
     util = u32_line(soa, abi.IDX_UTILITY)
@@ -205,7 +208,7 @@ def dynamic_scene(seed: int = 7, gx: int = 48, gy: int = 40, *, speed: float = 3
     flags = abi.FLAG_COHESION_LINE if with_cohesion else 0
     if with_cohesion:
         soa[abi.IDX_COHESION] = np.exp(np.log(1e5) + 0.5 * rng.standard_normal(n)).astype(np.float32)
-    prm = sea_ice_params(gx, gy, frame_steps=10, flags=flags)
+    prm = sea_ice_params(gx, gy, frame_steps=10, flags=flags, cellsize=cellsize)
     prm.ParticleVolume = prm.cellsize ** 2 * gx * gy / n_gen
     N = gx * gy
     current = None

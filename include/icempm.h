@@ -149,7 +149,9 @@ int ice_g2p(IceHandle *h, int recordPQ, int enablePointTransfer, int applyGlensL
 int ice_substep(IceHandle *h, double simulation_time, int recordPQ);
 /* `nsteps` substeps starting at step index `first_step` (t = dt*step),
  * recordPQ on steps that are multiples of UpdateEveryNthStep; replays a
- * captured HIP graph when use_graph != 0. */
+ * captured HIP graph of 16 substeps when use_graph != 0, except on the naive
+ * path (ICE_FLAG_NAIVE, the reference's launch structure), which always
+ * launches eagerly. */
 int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph);
 
 /* ---- frame boundary */

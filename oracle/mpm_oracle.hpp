@@ -10,6 +10,11 @@
 // reference file:line it follows.  Two additions beyond the reference are restated
 // here too, since the reference does not have them: the wind-stress term in
 // the node update and the optional per-particle cohesion (DP threshold).
+// One case the reference leaves undefined is given the HIP library's meaning:
+// a point whose 3x3 stencil leaves the grid (cell x or y at 0 or at the last
+// node) raises error_code_point_left_area, as the reference does
+// (kernels.cuh:102-103), but is skipped by P2G and G2P, where the reference
+// reads and writes nodes outside the grid arrays.
 //
 // P2G is deterministic for any thread count: particles are binned into
 // x-stripes of 4 columns; even stripes run in parallel, then odd stripes, and
@@ -339,6 +344,10 @@ template <class T> struct State {
         Mat2<T> Bp{B00[i], B01[i], B10[i], B11[i]};
         const T particle_mass = T(d.ParticleMass) * thick[i];
         const int cx = (int)(cell[i] & 0xffff), cy = (int)(cell[i] >> 16);
+        if (cx < 1 || cy < 1 || cx > gx - 2 || cy > gy - 2) {   // kernels.cuh:102-103; no scatter (see above)
+            err |= ICE_ERR_POINT_LEFT_AREA;
+            return;
+        }
         Mat2<T> PFt = kirchhoff(Fe, mu, kappa);
         const T k = T(d.dt_vol_Dpinv) * Jp[i] * thick[i];
         Mat2<T> st{particle_mass * Bp.a00 - k * PFt.a00, particle_mass * Bp.a01 - k * PFt.a01,
@@ -360,7 +369,6 @@ template <class T> struct State {
                 if (std::isnan(incVx) || std::isnan(incVy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
                 if (std::isnan(incM)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
             }
-        if (cx < 1 || cy < 1 || cx > gx - 2 || cy > gy - 2) err |= ICE_ERR_POINT_LEFT_AREA;
     }
 
     void p2g() {
@@ -449,6 +457,10 @@ template <class T> struct State {
             Mat2<T> Fe{F00[i], F01[i], F10[i], F11[i]};
             T jp = Jp[i];
             int cx = (int)(cell[i] & 0xffff), cy = (int)(cell[i] >> 16);
+            if (cx < 1 || cy < 1 || cx > gx - 2 || cy > gy - 2) {   // stencil outside the grid: skipped (see above)
+                err |= ICE_ERR_POINT_LEFT_AREA;
+                continue;
+            }
             T wx[3], wy[3];
             weights(x, wx);
             weights(y, wy);

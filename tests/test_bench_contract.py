@@ -1,8 +1,9 @@
-"""bench.py's contract, on the CPU: the roofline's algorithmic bytes are the
-two-pass substep's per-unit figure (168 B per point with the cohesion line,
-164 B without, plus 33 B per node), the measured HBM traffic comes from the
-committed counter summary, and a multi-GPU request exits with a message
-instead of running."""
+"""bench.py's contract, on the CPU: the dominant kernel (the G2P2G tile pass)
+is priced at the fused design's per-unit bytes (104 B per point with the
+cohesion line, 100 B without, plus 33 B per node), the two-pass substep's
+figure (168 B per point + 33 B per node) stays available beside it, the
+measured HBM traffic comes from the committed counter summary, and a
+multi-GPU request exits with a message instead of running."""
 import os
 import subprocess
 import sys
@@ -15,7 +16,19 @@ sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
 
-def test_algorithmic_bytes_are_the_two_pass_substep():
+def test_fused_kernel_is_priced_at_the_g2p2g_bytes():
+    n, nodes = 31_782_623, 4096 * 2048
+    B = bench.algorithmic_bytes(n, nodes, True)
+    # G2P reads 11 point slots (44 B) and writes 15 (60 B); P2G's point inputs
+    # come from registers; per node the P2G writes 12 B and the update reads 21 B
+    assert B["fused"] == 104 * n + 33 * nodes
+    assert bench.algorithmic_bytes(n, nodes, False)["fused"] == 100 * n + 33 * nodes
+    assert B["fused"] / 1e9 == pytest.approx(3.582, abs=1e-3)
+    # 0.6 of 8 TB/s on these bytes means a launch of at most 0.746 ms
+    assert B["fused"] / (0.6 * 8e12) * 1e3 == pytest.approx(0.746, abs=1e-3)
+
+
+def test_algorithmic_bytes_of_the_two_pass_substep():
     n, nodes = 31_880_000, 4096 * 2048
     B = bench.algorithmic_bytes(n, nodes, True)
     assert B["substep"] == 168 * n + 33 * nodes

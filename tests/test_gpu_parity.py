@@ -220,3 +220,76 @@ def test_sort_matches_stable_sort():
     np.testing.assert_array_equal(after.view(np.uint32), before[:, perm].view(np.uint32))
     assert dis2 == 0
     sim.close()
+
+
+def _scene_h215(kind):
+    """Cell size 215 m (the reference's production grids have cells of hundreds
+    of metres, input/n2k/nares2k.json:8).  "fast": the swirling scene scaled to
+    the cell size (velocities * h; cell crossings as at h = 1, the P2G momentum
+    dominated by m v).  "stress": slow motion (8e-3 m/s, Bp ~ 4e-5 1/s) so
+    that the P2G stress term and the APIC term each carry ~30 % of the node
+    momentum; the h factors of the stress term (dpos * h, dt_vol_Dpinv ~ 1/h^2,
+    kernels.cuh:76,86), of Bp (4/h, :230) and of the advection (dt/h, :240)
+    then all show."""
+    if kind == "fast":
+        return scenes.dynamic_scene(seed=13, cellsize=215.0)
+    return scenes.dynamic_scene(seed=14, cellsize=215.0, speed=3.8e-5, bp_scale=4e-5)
+
+
+@pytest.mark.parametrize("kind", ["fast", "stress"])
+@pytest.mark.parametrize("mode", ["naive", "pipelined", "unfused", "det"])
+def test_one_substep_cellsize_215(mode, kind):
+    _print(f"test_one_substep_cellsize_215 mode={mode} kind={kind}")
+    sc = _scene_h215(kind)
+    flags = sc.params.flags | {"naive": abi.FLAG_NAIVE, "det": abi.FLAG_DETERMINISTIC}.get(mode, 0)
+    sim = Simulation(sc, flags=flags, options=UNFUSED if mode == "unfused" else None)
+    sim.substep(0.0, True)
+    pts, forces, err, dis = sim.download()
+    mpp, vel = sim.grid()
+    o = oracle.Oracle.from_scene(sc, 64)
+    o.substep(0.0, True)
+    rep = compare_points(pts, o.points())
+    _print(rep)
+    assert err == o.error
+    assert dis == o.disabled
+    rmpp, rvel, frc = o.grid()
+    np.testing.assert_allclose(forces, frc, rtol=1e-5, atol=1e-6 * max(np.abs(frc).max(), 1.0))
+    names = ["m"] if mode == "naive" else ["m", "px", "py"]   # the naive update overwrites px, py
+    for k, name in enumerate(names):
+        e = np.abs(mpp[k] - rmpp[k]).max() / np.abs(rmpp[k]).max()
+        _print(name, e)
+        assert e < 2e-6, name
+    e = np.abs(vel - rvel).max() / np.abs(rvel).max()
+    _print("node velocity", e)
+    assert e < 2e-6
+    sim.close()
+
+
+@pytest.mark.parametrize("pipelined", [1, 0])
+def test_many_substeps_cellsize_215(pipelined):
+    """60 substeps of the fast swirling scene at h = 215 m: crossings, off-tile
+    points, disabled points, against the FP64 oracle at the many-substep bar."""
+    _print(f"test_many_substeps_cellsize_215 pipelined={pipelined}")
+    sc = scenes.dynamic_scene(seed=22, speed=500.0, gx=40, gy=36, fill=0.95, cellsize=215.0)
+    sc.params = sc.params.replace(waterDragEffectiveQuadratic=0.0)
+    sim = Simulation(sc, options={"pipelined": pipelined})
+    nsteps = 60
+    for k in range(nsteps):
+        sim.substep(DT * k, k == nsteps - 1)
+    info = sim.info()
+    pts, forces, err, dis = sim.download()
+    o2 = oracle.Oracle.from_scene(sc, 64)
+    for k in range(nsteps):
+        o2.substep(DT * k, k == nsteps - 1)
+    ref = o2.points()
+    cells_moved = np.mean(u32(by_point(ref), abi.IDX_CELL) != u32(by_point(sc.soa), abi.IDX_CELL))
+    _print("moved fraction", cells_moved, "offtile", info.n_offtile, "disabled", dis, o2.disabled)
+    assert cells_moved > 0.05
+    rep = compare_points(pts, ref, fields=[
+        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
+        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
+        ("Jp_inv", [abi.IDX_JP_INV], 1e-3)], max_bad_frac=0.01, max_flag_mismatch=0.01)
+    _print(rep)
+    assert dis == o2.disabled and err == o2.error
+    sim.close()

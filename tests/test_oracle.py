@@ -279,12 +279,127 @@ def test_p2g_single_point_brute_force():
             assert mpp[0, node] == pytest.approx(w * m, rel=1e-14)
 
 
-def test_g2p_reproduces_affine_velocity_field():
-    """APIC with quadratic B-splines is exact for v(x) = a + B x."""
+def _one_point_p2g(h, v, Bp, F, jp, th, pos=(0.23, -0.31), cell=(5, 6), n=12):
+    """The oracle's P2G of one point on an n x n grid of cell size h.
+    Returns (params, node momentum [2][N], node mass [N], node positions [2][N]
+    in metres, point position in metres, point mass)."""
+    prm = sea_ice_params(n, n, cellsize=h)
+    prm.ParticleVolume = 0.3 * h * h
+    soa = scenes.make_points(np.array([cell[0] + pos[0]]), np.array([cell[1] + pos[1]]), n, n)
+    soa[abi.IDX_VELX:abi.IDX_VELX + 2, 0] = v
+    soa[abi.IDX_BP00:abi.IDX_BP00 + 4, 0] = np.ravel(Bp)
+    soa[abi.IDX_FE00:abi.IDX_FE00 + 4, 0] = np.ravel(F)
+    soa[abi.IDX_JP_INV, 0] = jp
+    soa[abi.IDX_THICKNESS, 0] = th
+    o = oracle.Oracle(prm, 64)
+    o.set_points(soa)
+    o.set_grid(np.full(n * n, 100, np.uint8))
+    o.reset_grid()
+    o.p2g()
+    mpp, _, _ = o.grid()
+    assert o.error == 0
+    X, Y = np.meshgrid(np.arange(n, dtype=float), np.arange(n, dtype=float), indexing="ij")
+    xn = np.stack([X.reshape(-1), Y.reshape(-1)]) * h
+    # the point as stored (float32 slots), in metres
+    c = int(soa[abi.IDX_CELL].view(np.uint32)[0])
+    xp = np.array([(c & 0xFFFF) + float(soa[abi.IDX_POSX, 0]), (c >> 16) + float(soa[abi.IDX_POSX + 1, 0])]) * h
+    sp = soa[:, 0].astype(np.float64)
+    m = prm.ParticleMass * sp[abi.IDX_THICKNESS]
+    return prm, mpp[1:3], mpp[0], xn, xp, m, sp
+
+
+@pytest.mark.parametrize("h", [1.0, 215.0])
+def test_p2g_single_point_momentum_closed_form(h):
+    """One point with nonzero v and Bp (not symmetric), a non-identity Fe,
+    Jp_inv != 1 and thickness != 1, at h = 1 m and h = 215 m.  Node (i, j)
+    gets w (m v + S (x_i - x_p)) with S = m Bp - dt V0 Jp_inv th D^-1 tau(Fe)
+    (MLS-MPM with the reference's surface-density volume V0 Jp_inv th,
+    D^-1 = 4/h^2 pinned by the weights' second moment, tau pinned by its own
+    tests).  Independently of that form, the grid momentum's moments must be
+    those of APIC + MLS forces: sum p_i = m v; sum p_i (x_i - x_p)^T =
+    (h^2/4) m Bp - dt V0 Jp_inv th tau (the stress's momentum moment carries
+    no h); angular momentum x_p x m v + (h^2/4) m (Bp_yx - Bp_xy) (tau is
+    symmetric, so stress exerts no torque).  Magnitudes are chosen so that the
+    velocity, APIC and stress terms are comparable at each h, and each planted
+    slip (Jp_inv or h dropped, Bp transposed, stress sign flipped) must fail."""
+    F = rot(0.4) @ np.array([[1.0 + 1.3e-3, 0.6e-3], [-0.2e-3, 1.0 - 0.9e-3]]) @ rot(-0.1)
+    jp, th = 0.62, 1.7
+    v = np.array([1.0, -2.0]) * 0.27 / h
+    Bp = np.array([[0.8, -1.5], [2.1, 0.4]]) * 0.27 / h ** 2
+    prm, p, mass, xn, xp, m, sp = _one_point_p2g(h, v, Bp, F, jp, th)
+    # the stored (float32) state is what the oracle reads
+    v = sp[abi.IDX_VELX:abi.IDX_VELX + 2]
+    Bp = sp[abi.IDX_BP00:abi.IDX_BP00 + 4].reshape(2, 2)
+    F = sp[abi.IDX_FE00:abi.IDX_FE00 + 4].reshape(2, 2)
+    jp, th = sp[abi.IDX_JP_INV], sp[abi.IDX_THICKNESS]
+    tau = oracle.kirchhoff(prm, F)
+    dt, V0 = prm.InitialTimeStep, prm.ParticleVolume
+    assert abs(tau[0, 1] - tau[1, 0]) < 1e-9 * np.abs(tau).max()
+    nodes = np.nonzero(mass)[0]
+    assert nodes.size == 9
+    dx = xn[:, nodes] - xp[:, None]
+    w = mass[nodes] / m
+
+    def closed(vv, BB, k_tau, hh):
+        S = m * BB - k_tau * tau
+        return w * (m * vv[:, None] + S @ (dx * (hh / h)))
+
+    S_coef = dt * V0 * (4.0 / h ** 2) * jp * th
+    expect = closed(v, Bp, S_coef, h)
+    scale = np.abs(expect).max()
+    np.testing.assert_allclose(p[:, nodes], expect, rtol=0, atol=1e-12 * scale)
+    # the three terms are comparable, so a slip in any one shows
+    terms = [np.abs(w * (m * v[:, None])).max(), np.abs(w * (m * Bp @ dx)).max(),
+             np.abs(w * (S_coef * tau @ dx)).max()]
+    assert min(terms) > 0.05 * max(terms), terms
+    for slip in (closed(v, Bp, dt * V0 * (4.0 / h ** 2) * th, h),        # Jp_inv dropped
+                 closed(v, Bp, dt * V0 * (4.0 / h ** 2) * jp, h),        # thickness dropped
+                 closed(v, Bp.T, S_coef, h),                              # Bp transposed
+                 closed(v, Bp, -S_coef, h)):                              # stress sign flipped
+        assert np.abs(slip - p[:, nodes]).max() > 1e-3 * scale
+    if h != 1.0:
+        assert np.abs(closed(v, Bp, S_coef, 1.0) - p[:, nodes]).max() > 1e-3 * scale   # h dropped in dpos
+    # moments of the grid momentum
+    np.testing.assert_allclose(p.sum(axis=1), m * v, rtol=1e-12, atol=1e-12 * scale)
+    M1 = p[:, nodes] @ dx.T
+    M1_expect = (h * h / 4.0) * m * Bp - dt * V0 * jp * th * tau
+    np.testing.assert_allclose(M1, M1_expect, rtol=0, atol=1e-11 * np.abs(M1_expect).max())
+    L = float(np.sum(xn[0] * p[1] - xn[1] * p[0]))
+    L_expect = xp[0] * m * v[1] - xp[1] * m * v[0] + (h * h / 4.0) * m * (Bp[1, 0] - Bp[0, 1])
+    assert L == pytest.approx(L_expect, rel=1e-10)
+
+
+@pytest.mark.parametrize("h", [1.0, 215.0])
+def test_p2g_stress_power_on_affine_fields(h):
+    """Weak-form consistency of the P2G stress term: for any affine grid
+    velocity v_i = A x_i, the momentum the stress puts on the grid does the
+    work sum_i dp_i . (A x_i) = -dt V0 Jp_inv th (tau : A) (the stress power
+    of the point over dt), whatever h is.  Point at rest with Bp = 0, so the
+    grid momentum is the stress term alone."""
+    F = np.array([[1.0 + 2.1e-3, -0.7e-3], [0.4e-3, 1.0 - 1.2e-3]])
+    jp, th = 1.31, 0.55
+    prm, p, mass, xn, xp, m, sp = _one_point_p2g(h, np.zeros(2), np.zeros((2, 2)), F, jp, th, pos=(-0.12, 0.44))
+    F = sp[abi.IDX_FE00:abi.IDX_FE00 + 4].reshape(2, 2)
+    jp, th = sp[abi.IDX_JP_INV], sp[abi.IDX_THICKNESS]
+    tau = oracle.kirchhoff(prm, F)
+    dt, V0 = prm.InitialTimeStep, prm.ParticleVolume
+    for A in (np.array([[1.0, 0], [0, 0]]), np.array([[0, 1.0], [0, 0]]), np.array([[0, 0], [1.0, 0]]),
+              np.array([[0, 0], [0, 1.0]]), np.array([[0.3, -1.1], [0.7, 2.0]])):
+        work = float(np.sum(p * (A @ xn)))
+        expect = -dt * V0 * jp * th * float(np.sum(tau * A))
+        assert work == pytest.approx(expect, rel=1e-9, abs=1e-12 * dt * V0 * np.abs(tau).max())
+
+
+@pytest.mark.parametrize("h", [1.0, 215.0])
+def test_g2p_reproduces_affine_velocity_field(h):
+    """APIC with quadratic B-splines is exact for v(x) = a + B x (x in metres):
+    the point gets v(x_p) and C = B at any cell size (the 4/h factor of the
+    reference's Bp, kernels.cuh:230), and moves by v dt / h cells (:240)."""
     sc = _tiny_scene(seed=1)
+    sc.params = sc.params.replace(cellsize=h)
     o = oracle.Oracle.from_scene(sc, 64)
     gx, gy = sc.params.GridXTotal, sc.params.GridYTotal
-    X, Y = np.meshgrid(np.arange(gx, dtype=float), np.arange(gy, dtype=float), indexing="ij")
+    X, Y = np.meshgrid(np.arange(gx, dtype=float) * h, np.arange(gy, dtype=float) * h, indexing="ij")
     a = np.array([0.3, -0.2])
     B = np.array([[0.01, -0.02], [0.015, 0.005]])
     vx = (a[0] + B[0, 0] * X + B[0, 1] * Y).reshape(-1)
@@ -294,12 +409,40 @@ def test_g2p_reproduces_affine_velocity_field():
     out = o.points_f64()
     pts = sc.soa.astype(np.float64)
     cell = sc.soa[abi.IDX_CELL].view(np.uint32)
-    xp = (cell & 0xFFFF) + pts[abi.IDX_POSX]
-    yp = (cell >> 16) + pts[abi.IDX_POSX + 1]
-    np.testing.assert_allclose(out[abi.IDX_VELX], a[0] + B[0, 0] * xp + B[0, 1] * yp, atol=1e-13)
-    np.testing.assert_allclose(out[abi.IDX_VELX + 1], a[1] + B[1, 0] * xp + B[1, 1] * yp, atol=1e-13)
+    xp = ((cell & 0xFFFF) + pts[abi.IDX_POSX]) * h
+    yp = ((cell >> 16) + pts[abi.IDX_POSX + 1]) * h
+    vpx, vpy = a[0] + B[0, 0] * xp + B[0, 1] * yp, a[1] + B[1, 0] * xp + B[1, 1] * yp
+    vscale = max(np.abs(vpx).max(), np.abs(vpy).max())
+    np.testing.assert_allclose(out[abi.IDX_VELX], vpx, atol=1e-13 * vscale)
+    np.testing.assert_allclose(out[abi.IDX_VELX + 1], vpy, atol=1e-13 * vscale)
     for k, (i, j) in enumerate([(0, 0), (0, 1), (1, 0), (1, 1)]):
-        np.testing.assert_allclose(out[abi.IDX_BP00 + k], B[i, j], atol=1e-13)   # C = B (h = 1)
+        np.testing.assert_allclose(out[abi.IDX_BP00 + k], B[i, j], atol=1e-12)   # C = B
+    dt = sc.params.InitialTimeStep
+    same = out[abi.IDX_CELL].astype(np.uint32) == cell
+    assert same.mean() > 0.99
+    np.testing.assert_allclose(out[abi.IDX_POSX][same], (pts[abi.IDX_POSX] + vpx * dt / h)[same], atol=1e-12)
+    np.testing.assert_allclose(out[abi.IDX_POSX + 1][same], (pts[abi.IDX_POSX + 1] + vpy * dt / h)[same],
+                               atol=1e-12)
+
+
+def test_update_clamp_scales_with_cell_size():
+    """The velocity clamp |v|^2 > 0.1 vmax^2 with vmax = h / (4 dt)
+    (kernels.cuh:172, parameters_sim.cpp:136) at h = 215 m."""
+    h = 215.0
+    prm = sea_ice_params(8, 8, cellsize=h).replace(waterDragEffectiveQuadratic=0.0)
+    prm.ParticleVolume = 0.25 * h * h
+    o = oracle.Oracle(prm, 64)
+    o.set_points(np.zeros((19, 0), np.float32))
+    o.set_grid(np.full(64, 100, np.uint8))
+    vlim = math.sqrt(0.1) * 0.25 * h / prm.InitialTimeStep
+    m = np.zeros(64); px = np.zeros(64)
+    m[1], px[1] = 1.0, 1.05 * vlim        # above: zeroed
+    m[2], px[2] = 1.0, 0.95 * vlim        # below: kept (no drag here)
+    o.set_grid_state(m=m, px=px)
+    o.update_nodes(0.0)
+    _, vel, _ = o.grid()
+    assert vel[0, 1] == 0.0 and vel[1, 1] == 0.0
+    assert vel[0, 2] == pytest.approx(0.95 * vlim, rel=1e-15)
 
 
 def test_update_nodes_closed_forms():
