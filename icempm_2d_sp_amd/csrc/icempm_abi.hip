@@ -115,6 +115,7 @@ struct IceHandle {
     int fr_cur = 0;
     bool frames_valid = false;     // fr[fr_cur] holds the P2G of the current point state
     bool last_fused = false;       // the last substep ran pipelined: its grid is fr[fr_cur ^ 1]
+    bool p2g_called = false;       // ice_p2g on the pipelined path: the grid is fr[fr_cur]
     bool fused_opt = true;
     bool persistent = false;          // option "persistent": tile passes as persistent launches
     int resort_drift = 30;            // option "resort_drift": re-sort when this % of points left their cell
@@ -370,6 +371,7 @@ int upload_tables(IceHandle *h) {
     }
     h->frames_valid = false;   // the frame layout follows the tile shape
     h->last_fused = false;
+    h->p2g_called = false;
     std::vector<int> order(nd);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h->dom[a].N > h->dom[b].N; });
@@ -669,6 +671,7 @@ void end_timing_set(IceHandle *h) {
 void frames_stale(IceHandle *h) {
     h->frames_valid = false;
     h->last_fused = false;
+    h->p2g_called = false;
 }
 
 SurrArg surr_arg(const IceHandle *h) {
@@ -760,7 +763,23 @@ int phase_pipelined(IceHandle *h, const StepArg &S) {
     if ((rc = launch_offtile(h, tk, S, G, Fc, Fn, 1))) return rc;
     h->fr_cur ^= 1;
     h->last_fused = true;
+    h->p2g_called = false;
     for (auto &D : h->dom) D.steps_since_sort++;
+    return ICE_OK;
+}
+
+// ice_p2g on the pipelined path: this substep's P2G is already in the frames
+// (the previous substep's tile pass wrote it from the new point state), or
+// the pipeline fill writes it now.  The update and G2P follow in ice_g2p as
+// one tile pass, which also writes the next substep's P2G.
+int phase_p2g_pipelined(IceHandle *h) {
+    if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
+    int rc;
+    for (int d = 0; d < h->ndom(); d++)
+        if (!h->dom[d].sorted && (rc = device_sort(h, d, false, nullptr))) return rc;
+    if ((rc = upload_tables(h))) return rc;
+    if (!h->frames_valid && (rc = pipeline_fill(h))) return rc;
+    h->p2g_called = true;
     return ICE_OK;
 }
 
@@ -823,6 +842,7 @@ int build_graph(IceHandle *h, int chunk) {
 void graph_launched(IceHandle *h, int chunk) {
     for (auto &D : h->dom) D.steps_since_sort += chunk;
     if (h->pipelined()) h->last_fused = true;   // chunk is even: fr_cur is unchanged
+    h->p2g_called = false;
 }
 
 bool grid_ok(int gx, int gy, std::string &why) {
@@ -1264,8 +1284,16 @@ int ice_reset_grid(IceHandle *h) {
     return ICE_OK;
 }
 
+// The phase calls, in the order GPU_Implementation5 issues them per substep
+// (gpu_implementation5.cpp:18-81).  On the pipelined path they defer: ice_p2g
+// makes sure the substep's P2G is in the frames, ice_update_nodes records the
+// time, and ice_g2p runs the one tile pass (update + G2P, then the next P2G),
+// so a caller driving the phases runs the same kernels as ice_substep.  The
+// naive, deterministic and unfused paths launch each phase's kernels.
 int ice_p2g(IceHandle *h) {
     if (!h) return ICE_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    if (h->pipelined()) return phase_p2g_pipelined(h);
     frames_stale(h);
     return phase_p2g(h);
 }
@@ -1274,7 +1302,10 @@ int ice_update_nodes(IceHandle *h, float simulation_time, float windSpeed, float
     if (!h) return ICE_E_ARG;
     (void)windSpeed;   // uniform wind of the reference's signature; per-node stress comes from ice_upload_forcing
     (void)windAngle;
+    HIPCHK(h, hipSetDevice(h->device));
     h->pending_t = simulation_time;
+    h->last_t = simulation_time;
+    if (h->pipelined()) return ICE_OK;   // fused into the tile pass of ice_g2p
     StepArg S{(double)simulation_time, 0, nullptr, 0};
     frames_stale(h);
     return phase_update(h, S);
@@ -1284,10 +1315,16 @@ int ice_g2p(IceHandle *h, int recordPQ, int enablePointTransfer, int applyGlensL
     if (!h) return ICE_E_ARG;
     (void)enablePointTransfer;   // single partition: no point transfer
     (void)applyGlensLaw;         // Glen's law is commented out in the reference (kernels.cuh:625-652)
+    HIPCHK(h, hipSetDevice(h->device));
     StepArg S{h->pending_t, recordPQ, nullptr, 0};
     h->last_t = h->pending_t;
-    frames_stale(h);
-    int rc = phase_g2p(h, S);
+    int rc;
+    if (h->pipelined()) {
+        rc = phase_pipelined(h, S);
+    } else {
+        frames_stale(h);
+        rc = phase_g2p(h, S);
+    }
     rec(h, 5);
     end_timing_set(h);
     return rc;
@@ -1453,12 +1490,14 @@ int ice_download_grid(IceHandle *h, float *mpp, float *velocity) {
     if (!h) return ICE_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
     const Domain &D = h->dom[0];
-    if (h->last_fused && (mpp || velocity)) {
-        // the last substep's node sums are in the frames it read (a tile
-        // change clears last_fused, so the frame layout is current)
+    if ((h->last_fused || h->p2g_called) && (mpp || velocity)) {
+        // the node sums are in the frames: those the last substep read, or
+        // after ice_p2g on the pipelined path those the next tile pass reads
+        // (a tile change clears both flags, so the frame layout is current)
         const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
+        const int src = h->p2g_called ? h->fr_cur : h->fr_cur ^ 1;
         hipLaunchKernelGGL(tk->to_grid, dim3(1024), dim3(256), 0, h->stream, (const DomainDesc *)h->d_desc, 0,
-                           (const float *)h->fr[h->fr_cur ^ 1], grid_lines(h));
+                           (const float *)h->fr[src], grid_lines(h));
         LAUNCHCHK(h, "k_frames_to_grid");
     }
     if (mpp)

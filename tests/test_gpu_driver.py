@@ -25,3 +25,89 @@ def test_model_driver_runs_frames():
     arrays = re.findall(r"frame \d+ arrays: ice nodes (\d+), mass on water nodes ([-0-9.e+]+)", res.stdout)
     assert len(arrays) == 3 and all(int(a[0]) > 0 and float(a[1]) > 0 for a in arrays)
     assert "model_driver done" in res.stdout
+
+
+def _write_scene(path, sc):
+    import numpy as np
+    gx, gy = sc.params.GridXTotal, sc.params.GridYTotal
+    with open(path, "wb") as f:
+        np.array([gx, gy, sc.nlines, sc.n], np.int32).tofile(f)
+        np.array([sc.params.ParticleVolume], np.float64).tofile(f)
+        np.ascontiguousarray(sc.status, np.uint8).tofile(f)
+        np.ascontiguousarray(sc.wind, np.float32).tofile(f)
+        np.ascontiguousarray(sc.soa, np.float32).tofile(f)
+
+
+def _read_points(path):
+    import numpy as np
+    with open(path, "rb") as f:
+        nl, n = np.fromfile(f, np.int32, 2)
+        return np.fromfile(f, np.float32, nl * n).reshape(nl, n)
+
+
+def test_model_driver_matches_oracle_on_the_floe(tmp_path):
+    """The 256x128 floe (60 % ice, 0.1 Pa wind) for 200 substeps, driven by
+    the model.cpp-shaped C++ driver through the phase calls (reset_grid, p2g,
+    update_nodes, g2p, record_timings, transfer_from_device per frame of 100
+    substeps): held to the FP32 oracle at the bar of the same scene run by
+    ice_run (tests/test_gpu_parity.py), and to ice_run itself."""
+    import numpy as np
+    import oracle
+    from icempm_2d_sp_amd import abi, scenes
+    from icempm_2d_sp_amd.gpu import Simulation
+    from parity import compare_points
+    print("test_model_driver_matches_oracle_on_the_floe", flush=True)
+    sc = scenes.floe_256x128()
+    scene_path, out_path = str(tmp_path / "floe.bin"), str(tmp_path / "out.bin")
+    _write_scene(scene_path, sc)
+    res = subprocess.run([build.DRIVER, "256", "128", "2", "100", "0.1", scene_path, out_path], capture_output=True,
+                         text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0
+    pts = _read_points(out_path)
+    assert pts.shape == (abi.N_LINES, sc.n)
+    o32 = oracle.Oracle.from_scene(sc, 32)
+    o32.run(0, 200)
+    rep = compare_points(pts, o32.points(), fields=[
+        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-6),
+        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-7),
+        ("Jp_inv", [abi.IDX_JP_INV], 1e-7)])
+    print("driver vs FP32 oracle", rep, flush=True)
+    sim = Simulation(sc)
+    sim.run(0, 200, graph=True)
+    pr, _, err, dis = sim.download()
+    sim.close()
+    rep = compare_points(pts, pr, fields=[
+        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-6),
+        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-5),
+        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-7)])
+    print("driver vs ice_run", rep, flush=True)
+    assert err == 0 and "err 0x0" in res.stdout
+
+
+def test_model_driver_runs_at_the_ice_run_rate():
+    """The phase calls run the pipelined tile pass (one launch pair per
+    substep, the same kernels as ice_run): on a 2048x1024 sheet (~5M points)
+    the driver's wall time per substep is within 25 % of ice_run's graph
+    replays on the same-sized floe."""
+    import re
+    import time
+    from icempm_2d_sp_amd import scenes
+    from icempm_2d_sp_amd.gpu import Simulation
+    print("test_model_driver_runs_at_the_ice_run_rate", flush=True)
+    res = subprocess.run([build.DRIVER, "2048", "1024", "2", "48", "0.1"], capture_output=True, text=True, timeout=300)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0
+    walls = [float(w) for w in re.findall(r"frame \d+ wall ([0-9.]+) ms per substep", res.stdout)]
+    assert len(walls) == 2
+    sim = Simulation(scenes.floe_256x128(gx=2048, gy=1024))
+    sim.run(0, 16, graph=True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(16, 48, graph=True)
+    sim.sync()
+    ms_run = (time.perf_counter() - t0) * 1000 / 48
+    sim.close()
+    print(f"driver {walls[1]:.4f} ms per substep, ice_run {ms_run:.4f} ms", flush=True)
+    assert walls[1] <= 1.25 * ms_run

@@ -57,3 +57,45 @@ def test_resume_from_snapshot_is_bit_identical(tmp_path, precision):
     A, Cc = by_point(pa), by_point(pc)
     assert A.shape == Cc.shape
     assert np.array_equal(A.view(np.uint32), Cc.view(np.uint32))
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_resume_from_snapshot_matches_oracle(tmp_path, precision):
+    """A run saved at substep 20 in the reference's format (double precision:
+    the reference's default t_PointReal), loaded into a new handle and run
+    20 more substeps on the default (pipelined) path, against the FP64 oracle
+    started from the same file: the file's step and time restart the clock
+    (SnapshotManager::ReadPointsFromSnapshot, snapshotmanager.cpp:337-355),
+    and the continuation matches at the many-substep bar."""
+    import oracle
+    from parity import compare_points
+    sc = scenes.dynamic_scene(seed=72, with_cohesion=True)
+    dt = sc.params.InitialTimeStep
+    print("resume vs oracle, precision", precision, flush=True)
+    a = Simulation(sc)
+    a.run(0, 20, graph=True)
+    pa, _, ea, _ = a.download()
+    a.close()
+    path = str(tmp_path / "s00020.h5")
+    h5io.snapshot_save(path, pa, 20, 20 * dt, precision)
+    info = h5io.snapshot_info(path)
+    assert info["precision"] == precision and info["n"] == pa.shape[1] and info["cohesion"]
+    soa, step, t = h5io.snapshot_load(path, abi.N_LINES_MAX)
+    assert (step, t) == (20, 20 * dt)
+    resumed = scenes.Scene(sc.params, soa, sc.status, sc.current, sc.wind, "resumed")
+    b = Simulation(resumed)
+    b.run(step, 20, graph=True)
+    pb, _, eb, db = b.download()
+    b.close()
+    o = oracle.Oracle.from_scene(resumed, 64)
+    every = sc.params.UpdateEveryNthStep
+    for k in range(step, step + 20):
+        o.substep(k * dt, every > 0 and k % every == 0)
+    rep = compare_points(pb, o.points(), fields=[
+        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
+        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
+        ("Jp_inv", [abi.IDX_JP_INV], 1e-3)], max_bad_frac=0.01, max_flag_mismatch=0.01)
+    print(rep, flush=True)
+    assert ea == eb == o.error == 0
+    assert db == o.disabled
