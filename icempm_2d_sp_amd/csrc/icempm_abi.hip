@@ -112,6 +112,7 @@ struct IceHandle {
     // frames of the pipelined path (two buffers, substep n reads fr[fr_cur])
     float *fr[2] = {nullptr, nullptr};
     size_t fr_cap = 0;             // floats per buffer
+    size_t fr_used = 0;            // floats per buffer of the current tile shape
     int fr_cur = 0;
     bool frames_valid = false;     // fr[fr_cur] holds the P2G of the current point state
     bool last_fused = false;       // the last substep ran pipelined: its grid is fr[fr_cur ^ 1]
@@ -297,6 +298,7 @@ struct TileKernels {
     P2GFn p2g, p2g_det;
     G2PFn g2p;
     PipeFn fill, fused, fused_surr, fill_persistent, fused_persistent;
+    PipeFn fused_buf, fused_surr_buf;   // 32-bit addressing (BufLines, 32-bit frame offsets)
     OffFn offtile;
     ToGridFn to_grid;
 };
@@ -304,7 +306,8 @@ struct TileKernels {
 #define TK(W, H)                                                                                          \
     {W, H, k_p2g_tiles<W, H, false>, k_p2g_tiles<W, H, true>, k_g2p_tiles<W, H>, k_tile_pipeline<W, H, false>, \
      k_tile_pipeline<W, H, true>, k_tile_pipeline<W, H, true, true>, k_tile_pipeline<W, H, false, false, true>, \
-     k_tile_pipeline<W, H, true, false, true>, k_offtile_pipeline<W, H>, k_frames_to_grid<W, H>}
+     k_tile_pipeline<W, H, true, false, true>, k_tile_pipeline<W, H, true, false, false, true>,             \
+     k_tile_pipeline<W, H, true, true, false, true>, k_offtile_pipeline<W, H>, k_frames_to_grid<W, H>}
 const TileKernels kTiles[] = {TK(8, 8),   TK(8, 16),  TK(16, 8),  TK(16, 16), TK(16, 32),
                               TK(32, 16), TK(32, 32), TK(32, 64), TK(64, 32), TK(64, 64)};
 #undef TK
@@ -369,6 +372,7 @@ int upload_tables(IceHandle *h) {
         HIPCHK(h, hipMalloc(&h->fr[1], fr_need * sizeof(float)));
         h->fr_cap = fr_need;
     }
+    h->fr_used = fr_need;
     h->frames_valid = false;   // the frame layout follows the tile shape
     h->last_fused = false;
     h->p2g_called = false;
@@ -750,7 +754,9 @@ int phase_pipelined(IceHandle *h, const StepArg &S) {
     float *Fn = h->fr[h->fr_cur ^ 1];
     if (h->ntiles) {
         const bool surr = h->mlp_deployed && h->mlp_have_params && h->mlp_have_norm;
-        PipeFn fn = surr ? tk->fused_surr : tk->fused;
+        // 32-bit addressing when every point slot and frame offset fits it
+        const bool buf = buf_lines_ok(h->pitch) && h->fr_used < (size_t(1) << 30);
+        PipeFn fn = surr ? (buf ? tk->fused_surr_buf : tk->fused_surr) : (buf ? tk->fused_buf : tk->fused);
         unsigned grid;
         uint32_t *queue;
         if ((rc = tile_launch_shape(h, fn, surr ? nullptr : tk->fused_persistent, grid, queue))) return rc;

@@ -39,9 +39,19 @@ struct Frame {
     static constexpr int NN = W * LP;                         // LDS floats per component
 };
 
-template <int TW, int TH>
-__device__ __forceinline__ size_t frame_offset(const DomainDesc &dd, int tx, int ty) {
-    return (size_t)(dd.frame_base + (uint64_t)tx * dd.nty + ty) * (3 * Frame<TW, TH>::N);
+// Offset (floats) of frame (tx, ty) of a domain.  IdxT = uint32_t when every
+// frame offset fits 32 bits (the host checks the frame buffer size), which
+// keeps the index arithmetic off the 64-bit path.
+template <int TW, int TH, class IdxT = size_t>
+__device__ __forceinline__ IdxT frame_offset(const DomainDesc &dd, int tx, int ty) {
+    return (IdxT)(dd.frame_base + (uint64_t)tx * dd.nty + ty) * (IdxT)(3 * Frame<TW, TH>::N);
+}
+
+// F[o]; with 32-bit offsets as a 32-bit byte offset from F (frames below 2^30
+// floats), which the loads take as a register offset to a scalar base.
+__device__ __forceinline__ float frame_ld(const float *__restrict__ F, size_t o) { return F[o]; }
+__device__ __forceinline__ float frame_ld(const float *__restrict__ F, uint32_t o) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(F) + (o << 2));
 }
 
 // Sum of node (x, y) of a domain over every frame that covers it, ascending tx
@@ -49,13 +59,13 @@ __device__ __forceinline__ size_t frame_offset(const DomainDesc &dd, int tx, int
 // candidate frames are always read (a covered-or-not flag selects the value,
 // uncovered candidates read the frame of the tile holding the node), so the
 // loads of several nodes can be in flight together.
-template <int TW, int TH>
+template <int TW, int TH, class IdxT = size_t>
 __device__ __forceinline__ void frame_node_sum(const float *__restrict__ F, const DomainDesc &dd, int x, int y,
                                                float &m, float &px, float &py) {
     using Fr = Frame<TW, TH>;
     m = px = py = 0.f;
     const int ta = (x + 2) / TW, ua = (y + 2) / TH;
-    const size_t home = frame_offset<TW, TH>(dd, x / TW, y / TH) + (x % TW + 2) * Fr::H + (y % TH + 2);
+    const IdxT home = frame_offset<TW, TH, IdxT>(dd, x / TW, y / TH) + (x % TW + 2) * Fr::H + (y % TH + 2);
 #pragma unroll
     for (int dx = 1; dx >= 0; dx--) {
         const int tx = ta - dx;
@@ -66,8 +76,8 @@ __device__ __forceinline__ void frame_node_sum(const float *__restrict__ F, cons
             const int ty = ua - dy;
             const int ly = y - ty * TH + 2;
             const bool ok = okx && ty >= 0 && ty < dd.nty && ly < Fr::H;
-            const size_t o = ok ? frame_offset<TW, TH>(dd, tx, ty) + lx * Fr::H + ly : home;
-            const float a = F[o], b = F[o + Fr::N], c = F[o + 2 * Fr::N];
+            const IdxT o = ok ? frame_offset<TW, TH, IdxT>(dd, tx, ty) + lx * Fr::H + ly : home;
+            const float a = frame_ld(F, o), b = frame_ld(F, (IdxT)(o + Fr::N)), c = frame_ld(F, (IdxT)(o + 2 * Fr::N));
             m += ok ? a : 0.f;
             px += ok ? b : 0.f;
             py += ok ? c : 0.f;
@@ -198,17 +208,18 @@ struct G2PRaw {
     float x, y, f00, f01, f10, f11, jp, th, dpt;
 };
 
-__device__ __forceinline__ void load_g2p_raw(const Lines &L, const DevParams &P, uint32_t i, G2PRaw &r) {
+template <class LT>
+__device__ __forceinline__ void load_g2p_raw(const LT &L, const DevParams &P, uint32_t i, G2PRaw &r) {
     r.u = L.u(ICE_IDX_UTILITY, i);
     r.cell = L.u(ICE_IDX_CELL, i);
-    r.x = L(ICE_IDX_POSX, i);
-    r.y = L(ICE_IDX_POSX + 1, i);
-    r.f00 = L(ICE_IDX_FE00, i);
-    r.f01 = L(ICE_IDX_FE00 + 1, i);
-    r.f10 = L(ICE_IDX_FE00 + 2, i);
-    r.f11 = L(ICE_IDX_FE00 + 3, i);
-    r.jp = L(ICE_IDX_JP_INV, i);
-    r.th = L(ICE_IDX_THICKNESS, i);
+    r.x = L.get(ICE_IDX_POSX, i);
+    r.y = L.get(ICE_IDX_POSX + 1, i);
+    r.f00 = L.get(ICE_IDX_FE00, i);
+    r.f01 = L.get(ICE_IDX_FE00 + 1, i);
+    r.f10 = L.get(ICE_IDX_FE00 + 2, i);
+    r.f11 = L.get(ICE_IDX_FE00 + 3, i);
+    r.jp = L.get(ICE_IDX_JP_INV, i);
+    r.th = L.get(ICE_IDX_THICKNESS, i);
     r.dpt = dp_threshold(P, L, i);
 }
 
@@ -244,7 +255,10 @@ struct SurrArg {
     float lj0, lj1, d0, d1, jp0, jp1;   // feature ranges it was trained on
 };
 
-template <int TW, int TH, bool FUSED, bool SURR = false, bool PERSIST = false>
+// BUF: 32-bit addressing of the points (BufLines) and frames, chosen by the
+// host when the point buffer and the frames are small enough (buf_lines_ok,
+// frames below 2^30 floats).
+template <int TW, int TH, bool FUSED, bool SURR = false, bool PERSIST = false, bool BUF = false>
 __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0, StepArg S, SurrArg SA,
                                                                 const DomainDesc *__restrict__ D,
                                                                 const TileRef *__restrict__ T, int ntiles, Lines L0,
@@ -253,6 +267,8 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                                                                 const float *__restrict__ Fc, float *__restrict__ Fn,
                                                                 uint32_t *queue) {
     using Fr = Frame<TW, TH>;
+    using LT = typename std::conditional<BUF, BufLines, Lines>::type;
+    using IdxT = typename std::conditional<BUF, uint32_t, size_t>::type;
     static_assert(kTileThreads % TH == 0 && TH <= kTileThreads, "a round holds whole tile columns");
     constexpr int RC = kTileThreads / TH;              // tile columns per round
     constexpr int NROUND = (TW + RC - 1) / RC;
@@ -288,7 +304,9 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
         P.gx = dd.gx;
         P.gy = dd.gy;
         P.has_current = dd.has_current;
-        const Lines L{L0.base + dd.pt_off, L0.pitch};
+        LT L;
+        if constexpr (BUF) L = BufLines::make(L0.base + dd.pt_off, L0.pitch);
+        else L = Lines{L0.base + dd.pt_off, L0.pitch};
         const uint32_t *__restrict__ cs = cs0 + dd.cs_base;
         const size_t nb = dd.node_base;
         const GridLines G{G0.m + nb, G0.px + nb, G0.py + nb, G0.cur_x + nb, G0.cur_y + nb, G0.fx + nb, G0.fy + nb,
@@ -315,7 +333,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                 const int k = min(it * kTileThreads + (int)threadIdx.x, Fr::N - 1);
                 const int lx = k / Fr::H, ly = k - (k / Fr::H) * Fr::H;
                 const int x = min(max(fx0 + lx, 0), P.gx - 1), y = min(max(fy0 + ly, 0), P.gy - 1);
-                frame_node_sum<TW, TH>(Fc, dd, x, y, nm[it], nmx[it], nmy[it]);
+                frame_node_sum<TW, TH, IdxT>(Fc, dd, x, y, nm[it], nmx[it], nmy[it]);
                 const size_t node = (size_t)x * P.gy + y;
                 nst[it] = G.status[node];
                 nwx[it] = G.wind_x[node];
@@ -425,21 +443,12 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                                 float wx[3], wy[3];
                                 weights(s.x, wx);
                                 weights(s.y, wy);
+                                float2 nv[3][3];
 #pragma unroll
-                                for (int a = -1; a <= 1; a++)
+                                for (int a = 0; a < 3; a++)
 #pragma unroll
-                                    for (int b = -1; b <= 1; b++) {
-                                        const float w = wx[a + 1] * wy[b + 1];
-                                        const float2 nv = s_v[(rx + a) * Fr::H + (ry + b)];
-                                        const float dx = (float)a - s.x, dy = (float)b - s.y;
-                                        pvx += w * nv.x;
-                                        pvy += w * nv.y;
-                                        const float f = P.four_h_inv * w;
-                                        C.a00 += f * (nv.x * dx);
-                                        C.a01 += f * (nv.x * dy);
-                                        C.a10 += f * (nv.y * dx);
-                                        C.a11 += f * (nv.y * dy);
-                                    }
+                                    for (int b = 0; b < 3; b++) nv[a][b] = s_v[(rx + a - 1) * Fr::H + (ry + b - 1)];
+                                g2p_gather(nv, s.x, s.y, wx, wy, P.four_h_inv, pvx, pvy, C);
                                 g2p_predict(P, s, pvx, pvy, C, err, vf);
                             }
                         }
@@ -552,16 +561,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                     const int sx = (int)(pc & 0xffu) - (cx - fx0), sy = (int)(pc >> 8) - (cy - fy0);
                     if ((sx | sy) != 0 && !moved_stencil<TW, TH>(t, sx, sy, cx - fx0, cy - fy0, wx, wy, s_acc, err))
                         continue;
-#pragma unroll
-                    for (int a = 0; a < 3; a++)
-#pragma unroll
-                        for (int b = 0; b < 3; b++) {
-                            float dm, dpx, dpy;
-                            p2g_contrib(t, a - 1, b - 1, wx[a] * wy[b], dm, dpx, dpy);
-                            am[a][b] += dm;
-                            ax[a][b] += dpx;
-                            ay[a][b] += dpy;
-                        }
+                    p2g_accumulate(t, wx, wy, am, ax, ay);
                     any = true;
                 }
                 if (any) {
@@ -646,23 +646,17 @@ __device__ __forceinline__ void offtile_point(const DevParams &P, const DomainDe
         float wx[3], wy[3];
         weights(s.x, wx);
         weights(s.y, wy);
-        float pvx = 0.f, pvy = 0.f;
-        M2 C{0.f, 0.f, 0.f, 0.f};
-        for (int a = -1; a <= 1; a++)
-            for (int b = -1; b <= 1; b++) {
-                const float w = wx[a + 1] * wy[b + 1];
-                float nvx, nvy, mx, my, m;
+        float2 nv[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                float mx, my, m;
                 bool wall;
-                frame_node_velocity<TW, TH>(Fc, dd, G, P, ramp, s.cx + a, s.cy + b, nvx, nvy, mx, my, wall, m);
-                const float dx = (float)a - s.x, dy = (float)b - s.y;
-                pvx += w * nvx;
-                pvy += w * nvy;
-                const float f = P.four_h_inv * w;
-                C.a00 += f * (nvx * dx);
-                C.a01 += f * (nvx * dy);
-                C.a10 += f * (nvy * dx);
-                C.a11 += f * (nvy * dy);
+                frame_node_velocity<TW, TH>(Fc, dd, G, P, ramp, s.cx + a - 1, s.cy + b - 1, nv[a][b].x, nv[a][b].y, mx,
+                                            my, wall, m);
             }
+        float pvx, pvy;
+        M2 C;
+        g2p_gather(nv, s.x, s.y, wx, wy, P.four_h_inv, pvx, pvy, C);
         FInv vf;
         g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);

@@ -84,23 +84,14 @@ __global__ __launch_bounds__(kTileThreads) void k_g2p_tiles(DevParams P0, StepAr
         float wx[3], wy[3];
         weights(s.x, wx);
         weights(s.y, wy);
-        float pvx = 0.f, pvy = 0.f;
-        M2 C{0.f, 0.f, 0.f, 0.f};
+        float2 nv[3][3];
 #pragma unroll
-        for (int a = -1; a <= 1; a++)
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int b = -1; b <= 1; b++) {
-                const float w = wx[a + 1] * wy[b + 1];
-                const float2 nv = s_v[(rx + a) * RH + (ry + b)];
-                const float dx = (float)a - s.x, dy = (float)b - s.y;
-                pvx += w * nv.x;
-                pvy += w * nv.y;
-                const float f = P.four_h_inv * w;
-                C.a00 += f * (nv.x * dx);
-                C.a01 += f * (nv.x * dy);
-                C.a10 += f * (nv.y * dx);
-                C.a11 += f * (nv.y * dy);
-            }
+            for (int b = 0; b < 3; b++) nv[a][b] = s_v[(rx + a - 1) * RH + (ry + b - 1)];
+        float pvx, pvy;
+        M2 C;
+        g2p_gather(nv, s.x, s.y, wx, wy, P.four_h_inv, pvx, pvy, C);
         FInv vf;
         g2p_finish(P, s, pvx, pvy, C, th, dpt, err, vf);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
@@ -154,27 +145,19 @@ __global__ __launch_bounds__(256) void k_g2p_offtile(DevParams P0, StepArg S, co
         float wx[3], wy[3];
         weights(s.x, wx);
         weights(s.y, wy);
-        float pvx = 0.f, pvy = 0.f;
-        M2 C{0.f, 0.f, 0.f, 0.f};
-        for (int a = -1; a <= 1; a++)
-            for (int b = -1; b <= 1; b++) {
-                const float w = wx[a + 1] * wy[b + 1];
-                const size_t node = (size_t)(s.cy + b) + (size_t)(s.cx + a) * P.gy;
+        float2 nv[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                const size_t node = (size_t)(s.cy + b - 1) + (size_t)(s.cx + a - 1) * P.gy;
                 float cx = 0.f, cy = 0.f;
                 if (P.has_current) { cx = ccx[node]; cy = ccy[node]; }
-                float nvx, nvy;
                 bool wall;
-                node_velocity(gm[node], gpx[node], gpy[node], st[node], cx, cy, wwx[node], wwy[node], ramp, P, nvx,
-                              nvy, wall);
-                const float dx = (float)a - s.x, dy = (float)b - s.y;
-                pvx += w * nvx;
-                pvy += w * nvy;
-                const float f = P.four_h_inv * w;
-                C.a00 += f * (nvx * dx);
-                C.a01 += f * (nvx * dy);
-                C.a10 += f * (nvy * dx);
-                C.a11 += f * (nvy * dy);
+                node_velocity(gm[node], gpx[node], gpy[node], st[node], cx, cy, wwx[node], wwy[node], ramp, P,
+                              nv[a][b].x, nv[a][b].y, wall);
             }
+        float pvx, pvy;
+        M2 C;
+        g2p_gather(nv, s.x, s.y, wx, wy, P.four_h_inv, pvx, pvy, C);
         FInv vf;
         g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
         if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);

@@ -153,22 +153,15 @@ __global__ __launch_bounds__(256) void k_g2p_naive(DevParams P, StepArg S, Lines
             float wx[3], wy[3];
             weights(s.x, wx);
             weights(s.y, wy);
-            float pvx = 0.f, pvy = 0.f;
-            M2 C{0.f, 0.f, 0.f, 0.f};
-            for (int a = -1; a <= 1; a++)
-                for (int b = -1; b <= 1; b++) {
-                    const float w = wx[a + 1] * wy[b + 1];
-                    const size_t node = (size_t)(s.cy + b) + (size_t)(s.cx + a) * P.gy;
-                    const float nvx = gvx[node], nvy = gvy[node];
-                    const float dx = (float)a - s.x, dy = (float)b - s.y;
-                    pvx += w * nvx;
-                    pvy += w * nvy;
-                    const float f = P.four_h_inv * w;
-                    C.a00 += f * (nvx * dx);
-                    C.a01 += f * (nvx * dy);
-                    C.a10 += f * (nvy * dx);
-                    C.a11 += f * (nvy * dy);
+            float2 nv[3][3];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    const size_t node = (size_t)(s.cy + b - 1) + (size_t)(s.cx + a - 1) * P.gy;
+                    nv[a][b] = make_float2(gvx[node], gvy[node]);
                 }
+            float pvx, pvy;
+            M2 C;
+            g2p_gather(nv, s.x, s.y, wx, wy, P.four_h_inv, pvx, pvy, C);
             FInv vf;
             g2p_finish(P, s, pvx, pvy, C, th, dp_threshold(P, L, i), err, vf);
             if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);

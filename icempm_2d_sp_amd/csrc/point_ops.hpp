@@ -52,7 +52,38 @@ struct Lines {
     }
     __device__ __forceinline__ uint32_t u(int line, uint32_t i) const { return __float_as_uint((*this)(line, i)); }
     __device__ __forceinline__ void set_u(int line, uint32_t i, uint32_t v) const { (*this)(line, i) = __uint_as_float(v); }
+    __device__ __forceinline__ float get(int line, uint32_t i) const { return (*this)(line, i); }
+    __device__ __forceinline__ void set(int line, uint32_t i, float v) const { (*this)(line, i) = v; }
 };
+
+// The same lines through one buffer resource over a domain's point range:
+// slot i of line l is byte i * 4 + l * pitch * 4, the line's part a scalar
+// offset (SALU), so the lines need no 64-bit base each.  Valid while
+// 20 * pitch * 4 < 2^32 (the host picks this form only then).
+struct BufLines {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t pitch4;
+    __device__ __forceinline__ static BufLines make(float *base, size_t pitch) {
+        BufLines b;
+        // word 3 of the gfx9 buffer descriptor for raw 32-bit accesses
+        b.r = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)0xffffffff, 0x00020000);
+        b.pitch4 = (uint32_t)(pitch * 4);
+        return b;
+    }
+    __device__ __forceinline__ uint32_t u(int line, uint32_t i) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i << 2), (int)((uint32_t)line * pitch4), 0);
+    }
+    __device__ __forceinline__ void set_u(int line, uint32_t i, uint32_t v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)(i << 2), (int)((uint32_t)line * pitch4), 0);
+    }
+    __device__ __forceinline__ float get(int line, uint32_t i) const { return __uint_as_float(u(line, i)); }
+    __device__ __forceinline__ void set(int line, uint32_t i, float v) const { set_u(line, i, __float_as_uint(v)); }
+};
+
+// Whether a point buffer of `pitch` slots per line can use BufLines.
+__host__ __device__ constexpr bool buf_lines_ok(size_t pitch) {
+    return (uint64_t)pitch * 4u * ICE_N_LINES_MAX <= 0xffffffffull;
+}
 
 // P2G inputs of one point: mass and the APIC affine momentum terms
 // (kernels.cuh:54-76).  Contribution to node offset (a, b):
@@ -65,18 +96,19 @@ struct P2GPoint {
     float s00, s01, s10, s11;   // subterm2 * h
 };
 
-__device__ __forceinline__ void load_p2g_point(const Lines &L, uint32_t i, uint32_t cell, const DevParams &P,
+template <class LT>
+__device__ __forceinline__ void load_p2g_point(const LT &L, uint32_t i, uint32_t cell, const DevParams &P,
                                                P2GPoint &q) {
     q.cx = (int)(cell & 0xffff);
     q.cy = (int)(cell >> 16);
-    q.x = L(ICE_IDX_POSX, i);
-    q.y = L(ICE_IDX_POSX + 1, i);
-    const float vx = L(ICE_IDX_VELX, i), vy = L(ICE_IDX_VELX + 1, i);
-    M2 F{L(ICE_IDX_FE00, i), L(ICE_IDX_FE00 + 1, i), L(ICE_IDX_FE00 + 2, i), L(ICE_IDX_FE00 + 3, i)};
-    const float b00 = L(ICE_IDX_BP00, i), b01 = L(ICE_IDX_BP00 + 1, i);
-    const float b10 = L(ICE_IDX_BP00 + 2, i), b11 = L(ICE_IDX_BP00 + 3, i);
-    const float Jp = L(ICE_IDX_JP_INV, i);
-    const float th = L(ICE_IDX_THICKNESS, i);
+    q.x = L.get(ICE_IDX_POSX, i);
+    q.y = L.get(ICE_IDX_POSX + 1, i);
+    const float vx = L.get(ICE_IDX_VELX, i), vy = L.get(ICE_IDX_VELX + 1, i);
+    M2 F{L.get(ICE_IDX_FE00, i), L.get(ICE_IDX_FE00 + 1, i), L.get(ICE_IDX_FE00 + 2, i), L.get(ICE_IDX_FE00 + 3, i)};
+    const float b00 = L.get(ICE_IDX_BP00, i), b01 = L.get(ICE_IDX_BP00 + 1, i);
+    const float b10 = L.get(ICE_IDX_BP00 + 2, i), b11 = L.get(ICE_IDX_BP00 + 3, i);
+    const float Jp = L.get(ICE_IDX_JP_INV, i);
+    const float th = L.get(ICE_IDX_THICKNESS, i);
     q.mass = P.particle_mass * th;
     q.mvx = q.mass * vx;
     q.mvy = q.mass * vy;
@@ -97,6 +129,59 @@ __device__ __forceinline__ void p2g_contrib(const P2GPoint &q, int a, int b, flo
     dm = w * q.mass;
     dpx = w * (q.mvx + q.s00 * dx + q.s01 * dy);
     dpy = w * (q.mvy + q.s10 * dx + q.s11 * dy);
+}
+
+// A point's whole 3 x 3 P2G stencil added to register sums, node offset
+// (a - 1, b - 1) in [a][b].  The affine momentum w (m v + S ((a-1) - x,
+// (b-1) - y)) is evaluated as w (A + S (a-1, b-1)) with A = m v - S (x, y),
+// and the mass term as (wx[a] m) wy[b], so each node costs three FMAs.
+__device__ __forceinline__ void p2g_accumulate(const P2GPoint &t, const float wx[3], const float wy[3],
+                                               float am[3][3], float ax[3][3], float ay[3][3]) {
+    const float Ax = t.mvx - (t.s00 * t.x + t.s01 * t.y);
+    const float Ay = t.mvy - (t.s10 * t.x + t.s11 * t.y);
+    const float cx[3] = {Ax - t.s00, Ax, Ax + t.s00};
+    const float cy[3] = {Ay - t.s10, Ay, Ay + t.s10};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float wm = wx[a] * t.mass;
+        const float vx[3] = {cx[a] - t.s01, cx[a], cx[a] + t.s01};
+        const float vy[3] = {cy[a] - t.s11, cy[a], cy[a] + t.s11};
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const float w = wx[a] * wy[b];
+            am[a][b] += wm * wy[b];
+            ax[a][b] += w * vx[b];
+            ay[a][b] += w * vy[b];
+        }
+    }
+}
+
+// The G2P gather of one point (kernels.cuh:224-236) from the velocities of its
+// 3 x 3 nodes, v[a][b] at offset (a - 1, b - 1), factored by the separable
+// weights w = wx[a] wy[b]:
+//   v_p = sum_a wx[a] sum_b wy[b] v[a][b]
+//   C   = (4/h) sum_ab w v ((a-1) - x, (b-1) - y)^T
+//       = (4/h) (sum_ab w v (a-1) - x v_p, sum_ab w v (b-1) - y v_p)
+// (sum_a wx[a] (a-1) = x for the quadratic B-spline weights).
+__device__ __forceinline__ void g2p_gather(const float2 v[3][3], float x, float y, const float wx[3],
+                                           const float wy[3], float four_h_inv, float &pvx, float &pvy, M2 &C) {
+    float sx[3], sy[3], tx[3], ty[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        sx[a] = wy[0] * v[a][0].x + wy[1] * v[a][1].x + wy[2] * v[a][2].x;
+        sy[a] = wy[0] * v[a][0].y + wy[1] * v[a][1].y + wy[2] * v[a][2].y;
+        tx[a] = wy[2] * v[a][2].x - wy[0] * v[a][0].x;
+        ty[a] = wy[2] * v[a][2].y - wy[0] * v[a][0].y;
+    }
+    pvx = wx[0] * sx[0] + wx[1] * sx[1] + wx[2] * sx[2];
+    pvy = wx[0] * sy[0] + wx[1] * sy[1] + wx[2] * sy[2];
+    const float ax = wx[2] * sx[2] - wx[0] * sx[0], ay = wx[2] * sy[2] - wx[0] * sy[0];
+    const float bx = wx[0] * tx[0] + wx[1] * tx[1] + wx[2] * tx[2];
+    const float by = wx[0] * ty[0] + wx[1] * ty[1] + wx[2] * ty[2];
+    C.a00 = four_h_inv * (ax - x * pvx);
+    C.a01 = four_h_inv * (bx - y * pvx);
+    C.a10 = four_h_inv * (ay - x * pvy);
+    C.a11 = four_h_inv * (by - y * pvy);
 }
 
 // Everything G2P does after the stencil gather (kernels.cuh:239-314).
@@ -167,8 +252,9 @@ __device__ __forceinline__ void g2p_finish(const DevParams &P, G2PState &s, floa
     if (s.u & kCrushed) g2p_return_map(P, s, thickness, dpt, err, vf);
 }
 
-__device__ __forceinline__ float dp_threshold(const DevParams &P, const Lines &L, uint32_t i) {
-    if (P.has_cohesion) return -L(ICE_IDX_COHESION, i) * P.inv_tan_phi;
+template <class LT>
+__device__ __forceinline__ float dp_threshold(const DevParams &P, const LT &L, uint32_t i) {
+    if (P.has_cohesion) return -L.get(ICE_IDX_COHESION, i) * P.inv_tan_phi;
     return P.DP_threshold_p;
 }
 
@@ -199,29 +285,30 @@ __device__ __forceinline__ void g2p_store(const Lines &L, uint32_t i, const G2PS
 // The pipelined path's store: position and Fe every substep; Jp_inv, the
 // utility word and the cell word when they changed; velocity and Bp (read by
 // no later substep of the pipeline) only when `complete`; P and Q when rec.
-__device__ __forceinline__ void g2p_store_pipelined(const Lines &L, uint32_t i, const G2PState &s, uint32_t u0,
+template <class LT>
+__device__ __forceinline__ void g2p_store_pipelined(const LT &L, uint32_t i, const G2PState &s, uint32_t u0,
                                                    float jp0, float pvx, float pvy, const M2 &C, bool rec,
                                                    bool complete) {
-    L(ICE_IDX_POSX, i) = s.x;
-    L(ICE_IDX_POSX + 1, i) = s.y;
-    L(ICE_IDX_FE00, i) = s.Fe.a00;
-    L(ICE_IDX_FE00 + 1, i) = s.Fe.a01;
-    L(ICE_IDX_FE00 + 2, i) = s.Fe.a10;
-    L(ICE_IDX_FE00 + 3, i) = s.Fe.a11;
-    if (__float_as_uint(s.Jp) != __float_as_uint(jp0)) L(ICE_IDX_JP_INV, i) = s.Jp;
+    L.set(ICE_IDX_POSX, i, s.x);
+    L.set(ICE_IDX_POSX + 1, i, s.y);
+    L.set(ICE_IDX_FE00, i, s.Fe.a00);
+    L.set(ICE_IDX_FE00 + 1, i, s.Fe.a01);
+    L.set(ICE_IDX_FE00 + 2, i, s.Fe.a10);
+    L.set(ICE_IDX_FE00 + 3, i, s.Fe.a11);
+    if (__float_as_uint(s.Jp) != __float_as_uint(jp0)) L.set(ICE_IDX_JP_INV, i, s.Jp);
     if (s.u != u0) L.set_u(ICE_IDX_UTILITY, i, s.u);
     if (s.cell_updated) L.set_u(ICE_IDX_CELL, i, ((uint32_t)s.cy << 16) | (uint32_t)s.cx);
     if (complete) {
-        L(ICE_IDX_VELX, i) = pvx;
-        L(ICE_IDX_VELX + 1, i) = pvy;
-        L(ICE_IDX_BP00, i) = C.a00;
-        L(ICE_IDX_BP00 + 1, i) = C.a01;
-        L(ICE_IDX_BP00 + 2, i) = C.a10;
-        L(ICE_IDX_BP00 + 3, i) = C.a11;
+        L.set(ICE_IDX_VELX, i, pvx);
+        L.set(ICE_IDX_VELX + 1, i, pvy);
+        L.set(ICE_IDX_BP00, i, C.a00);
+        L.set(ICE_IDX_BP00 + 1, i, C.a01);
+        L.set(ICE_IDX_BP00 + 2, i, C.a10);
+        L.set(ICE_IDX_BP00 + 3, i, C.a11);
     }
     if (rec) {
-        L(ICE_IDX_P, i) = s.p_tr;
-        L(ICE_IDX_Q, i) = s.q_tr;
+        L.set(ICE_IDX_P, i, s.p_tr);
+        L.set(ICE_IDX_Q, i, s.q_tr);
     }
 }
 
