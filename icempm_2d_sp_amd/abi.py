@@ -64,6 +64,7 @@ class IceInfo(C.Structure):
 
 
 MLP_PARAMS = 4675
+MLP_EVAL_NOALLOC_MAX_ROWS = 65536
 
 
 class IceMlpNorm(C.Structure):
@@ -133,6 +134,7 @@ SIGNATURES = {
     "ice_mlp_forward": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_void_p]),
     "ice_mlp_grad": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]),
     "ice_mlp_eval": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]),
+    "ice_mlp_eval_noalloc": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]),
     "ice_mlp_deploy": (C.c_int, [_P, C.c_int, C.c_float]),
     "ice_mlp_counts": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "ice_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),

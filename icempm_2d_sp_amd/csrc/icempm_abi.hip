@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -2017,6 +2018,62 @@ int ice_mlp_eval(IceHandle *h, const float *x, uint32_t n, float tol, float *out
     (void)hipFree(dx);
     if (dout) (void)hipFree(dout);
     if (e != hipSuccess) { h->err = std::string("ice_mlp_eval: ") + hipGetErrorString(e); return ICE_E_HIP; }
+    return ICE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Staging of ice_mlp_eval_noalloc: ICE_MLP_EVAL_NOALLOC_MAX_ROWS rows of
+// features (4 floats) then as many rows of results (6 floats), 2.6 MB.  Each
+// GPU holds its own copy until the process ends; every call on a GPU shares
+// it (calls that overlap on one GPU race).
+constexpr size_t kEvalRows = ICE_MLP_EVAL_NOALLOC_MAX_ROWS;
+__device__ __align__(256) float scratch[kEvalRows * (kMlpIn + 6)];
+
+// The array's device address on each GPU, looked up once per device with that
+// device current (host code never uses the symbol's own name as an address).
+constexpr int kMaxDevices = 64;
+float *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for(IceHandle *h, float **out) {
+    if (h->device < 0 || h->device >= kMaxDevices) { h->err = "device index beyond the scratch table"; return ICE_E_DEVICE; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[h->device]) {
+        HIPCHK(h, hipSetDevice(h->device));
+        void *p = nullptr;
+        HIPCHK(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[h->device] = static_cast<float *>(p);
+    }
+    *out = g_scratch[h->device];
+    return ICE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ice_mlp_eval_noalloc(IceHandle *h, const float *x, uint32_t n, float tol, float *out) {
+    if (!h || (!x && n) || (!out && n)) return ICE_E_ARG;
+    if (n > kEvalRows) {
+        h->err = "ice_mlp_eval_noalloc: more than ICE_MLP_EVAL_NOALLOC_MAX_ROWS (65536) rows";
+        return ICE_E_CAPACITY;
+    }
+    int rc = mlp_ready(h);
+    if (rc || n == 0) return rc;
+    float *S = nullptr;
+    if ((rc = scratch_for(h, &S))) return rc;
+    HIPCHK(h, hipSetDevice(h->device));
+    float *dx = S, *dout = S + kEvalRows * kMlpIn;
+    HIPCHK(h, hipMemcpyAsync(dx, x, sizeof(float) * (size_t)n * kMlpIn, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks_for(n, 64)), dim3(64), 0, h->stream,
+                       MlpDev{h->mlp_pk, h->mlp_p, h->mlp_ipk}, h->mlp_norm, h->dp, mlp_phys(h), (const float *)dx, n,
+                       tol, dout);
+    LAUNCHCHK(h, "k_mlp_eval");
+    HIPCHK(h, hipMemcpyAsync(out, dout, sizeof(float) * (size_t)n * 6, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return ICE_OK;
 }
 

@@ -322,6 +322,15 @@ int ice_mlp_grad(IceHandle *h, const float *x, const float *t, uint32_t n, float
  * (p', q'), yield violation |q' - q_yield(p')| / q_yield(p') of the
  * surrogate's state, 1 if the surrogate is accepted at tolerance `tol`. */
 int ice_mlp_eval(IceHandle *h, const float *x, uint32_t n, float tol, float *out);
+/* No counterpart in the reference.  ice_mlp_eval for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernel and results, with the
+ * rows staged through a fixed device array of ICE_MLP_EVAL_NOALLOC_MAX_ROWS
+ * rows (2.6 MB per GPU, held until the process ends) instead of buffers
+ * allocated per call.  n beyond that returns ICE_E_CAPACITY.  Every call on a
+ * GPU shares the array: calls that overlap on one GPU race. */
+#define ICE_MLP_EVAL_NOALLOC_MAX_ROWS 65536
+int ice_mlp_eval_noalloc(IceHandle *h, const float *x, uint32_t n, float tol, float *out);
 /* Use the surrogate for the plastic projection in pipelined substeps
  * (enable != 0), falling back to the exact map when its yield violation
  * exceeds `tol`, its features leave the training ranges or thickness != 1.
