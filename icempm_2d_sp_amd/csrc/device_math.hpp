@@ -16,6 +16,13 @@
 // singular vectors is ((s0+s1)/2) Rot + ((s0-s1)/2) Refl, so the return map
 // never forms U and V.  J - 1 is formed from e = F - I, and dev(F F^T) from
 // (D, G, H), so stresses keep the relative accuracy of the stored F.
+//
+// The device stores e = Fe - I in the four Fe lines (upload subtracts I,
+// download adds it back), not Fe: near Fe = I the spacing of FP32 is 6e-8,
+// while one substep of a slow floe changes the strain by 1e-10 or less, which
+// an FP32 Fe would round away.  invariants() reads e, the F update adds
+// dt C (I + e), and rebuild() writes e' from the new singular values minus
+// one, so no step of the substep forms 1 + (small) and subtracts 1 again.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -106,16 +113,16 @@ struct FInv {
     float iJ;           // 1 / J
 };
 
-__device__ __forceinline__ FInv invariants(const M2 &F) {
+// The invariants of F = I + e from e.
+__device__ __forceinline__ FInv invariants(const M2 &e) {
     FInv v;
-    const float e00 = F.a00 - 1.f, e11 = F.a11 - 1.f;
-    v.E = 0.5f * (F.a00 + F.a11);
-    v.H = 0.5f * (F.a10 - F.a01);
-    v.D = 0.5f * (e00 - e11);
-    v.G = 0.5f * (F.a01 + F.a10);
+    v.E = 1.f + 0.5f * (e.a00 + e.a11);
+    v.H = 0.5f * (e.a10 - e.a01);
+    v.D = 0.5f * (e.a00 - e.a11);
+    v.G = 0.5f * (e.a01 + e.a10);
     v.Q = sqrt_hw(v.E * v.E + v.H * v.H);
     v.R = sqrt_hw(v.D * v.D + v.G * v.G);
-    v.Jm1 = e00 + e11 + (e00 * e11 - F.a01 * F.a10);
+    v.Jm1 = e.a00 + e.a11 + (e.a00 * e.a11 - e.a01 * e.a10);
     v.J = 1.f + v.Jm1;
     v.iJ = rcp_nr(v.J);
     return v;
@@ -158,32 +165,42 @@ __device__ __forceinline__ float q_yield_at(float p, float dpt, const DevParams 
     return fminf(q_fs, q_dp);
 }
 
-// F' = ((s0+s1)/2) Rot + ((s0-s1)/2) Refl for new singular values (s0, s1).
-__device__ __forceinline__ M2 rebuild(const FInv &v, float s0, float s1) {
-    const float sb = 0.5f * (s0 + s1), sd = 0.5f * (s0 - s1);
-    float rc = 1.f, rs = 0.f;                  // Rot = I when Q = 0 (helper_math.cuh:31-39)
-    if (v.Q != 0.f) { const float iq = rcp_nr(v.Q); rc = v.E * iq; rs = v.H * iq; }
+// e' = F' - I for F' = ((s0+s1)/2) Rot + ((s0-s1)/2) Refl, from the new
+// singular values minus one, m0 = s0 - 1 and m1 = s1 - 1.  The diagonal of
+// ((s0+s1)/2) Rot - I is ((m0+m1)/2) cos + (cos - 1), with cos - 1 = (E - Q)/Q
+// = -H^2 / ((E + Q) Q) for E > 0, so nothing cancels.
+__device__ __forceinline__ M2 rebuild(const FInv &v, float m0, float m1) {
+    const float sbm1 = 0.5f * (m0 + m1), sd = 0.5f * (m0 - m1);
+    float rc = 1.f, rcm1 = 0.f, rs = 0.f;      // Rot = I when Q = 0 (helper_math.cuh:31-39)
+    if (v.Q != 0.f) {
+        const float iq = rcp_nr(v.Q);
+        rc = v.E * iq;
+        rs = v.H * iq;
+        rcm1 = (v.E > 0.f ? -v.H * v.H * rcp_nr(v.E + v.Q) : v.E - v.Q) * iq;
+    }
     float fc = 0.f, fs = 0.f;
     if (v.R != 0.f) { const float ir = rcp_nr(v.R); fc = v.D * ir; fs = v.G * ir; }
-    M2 F;
-    F.a00 = sb * rc + sd * fc;
-    F.a01 = -sb * rs + sd * fs;
-    F.a10 = sb * rs + sd * fs;
-    F.a11 = sb * rc - sd * fc;
-    return F;
+    M2 e;
+    e.a00 = sbm1 * rc + rcm1 + sd * fc;
+    e.a01 = -(sbm1 * rs + rs) + sd * fs;
+    e.a10 = (sbm1 * rs + rs) + sd * fs;
+    e.a11 = sbm1 * rc + rcm1 - sd * fc;
+    return e;
 }
 
-// Wolper_Drucker_Prager (kernels.cuh:374-483) on the invariants.  Returns the
-// case: 0 tension with Jp_inv < 0.1, 1 tension, 2 elastic, 3 plastic.
+// Wolper_Drucker_Prager (kernels.cuh:374-483) on the invariants; e holds
+// Fe - I.  Returns the case: 0 tension with Jp_inv < 0.1, 1 tension, 2
+// elastic, 3 plastic.  Each new singular value s enters rebuild() as
+// s - 1 = (s^2 - 1) / (s + 1).
 __device__ __forceinline__ int return_map(const FInv &v, float p, float q, float thickness, float dpt,
-                                          const DevParams &P, M2 &Fe, float &Jp_inv) {
+                                          const DevParams &P, M2 &e, float &Jp_inv) {
     if (p < dpt) {
         if (Jp_inv < 0.1f) {
-            const float s = sqrt_hw(v.J);
-            Fe = rebuild(v, s, s);
+            const float m = v.Jm1 * rcp_nr(sqrt_hw(v.J) + 1.f);   // sqrt(J) - 1
+            e = rebuild(v, m, m);
             return 0;
         }
-        Fe = rebuild(v, 1.f, 1.f);
+        e = rebuild(v, 0.f, 0.f);
         Jp_inv *= v.iJ;
         return 1;
     }
@@ -192,10 +209,13 @@ __device__ __forceinline__ int return_map(const FInv &v, float p, float q, float
     const float p1 = fminf(p, p_ridge_max);
     const float q1 = q_yield_at(p1, dpt, P);
     const float Je_new = sqrt_hw(1.f - p1 * P.inv_half_kappa);
+    const float jm = -p1 * P.inv_half_kappa * rcp_nr(Je_new + 1.f);   // Je_new - 1
     // normalized(s_hat) = sign(J) (1, -1)/sqrt2 when sigma0^2 != sigma1^2, else 0
     float t = 0.f;
     if (v.Q * v.R != 0.f) t = q1 * fabsf(v.J) * P.inv_2mu;
-    Fe = rebuild(v, sqrt_hw(Je_new + t), sqrt_hw(Je_new - t));
+    const float m0 = (jm + t) * rcp_nr(sqrt_hw(Je_new + t) + 1.f);
+    const float m1 = (jm - t) * rcp_nr(sqrt_hw(Je_new - t) + 1.f);
+    e = rebuild(v, m0, m1);
     Jp_inv *= Je_new * v.iJ;
     return 3;
 }

@@ -143,6 +143,10 @@ struct IceHandle {
     uint32_t mlp_adam_t = 0;
     bool mlp_deployed = false;
     float mlp_tol = 1e-4f;
+    // recording of ridge-capped plastic trial states (ice_mlp_record)
+    float *mlp_rec = nullptr;
+    unsigned long long *mlp_rec_cnt = nullptr;   // [0] seen, [1] kept
+    uint32_t mlp_rec_cap = 0, mlp_rec_keep = 0;
 
     int ndom() const { return (int)dom.size(); }
     float *gline(int l) const { return grid + (size_t)l * gpitch; }
@@ -679,8 +683,18 @@ void frames_stale(IceHandle *h) {
     h->p2g_called = false;
 }
 
+bool surr_infer(const IceHandle *h) { return h->mlp_deployed && h->mlp_have_params && h->mlp_have_norm; }
+// The tile pass with the learned projection runs when it is deployed or while
+// trial states are recorded.
+bool surr_pass(const IceHandle *h) { return surr_infer(h) || h->mlp_rec != nullptr; }
+
 SurrArg surr_arg(const IceHandle *h) {
     SurrArg a{};
+    a.infer = surr_infer(h) ? 1 : 0;
+    a.rec_x = h->mlp_rec;
+    a.rec_count = h->mlp_rec_cnt;
+    a.rec_cap = h->mlp_rec_cap;
+    a.rec_keep = h->mlp_rec_keep;
     a.M = MlpDev{h->mlp_pk, h->mlp_p, h->mlp_ipk};
     a.N = h->mlp_norm;
     a.tol = h->mlp_tol;
@@ -754,7 +768,7 @@ int phase_pipelined(IceHandle *h, const StepArg &S) {
     const float *Fc = h->fr[h->fr_cur];
     float *Fn = h->fr[h->fr_cur ^ 1];
     if (h->ntiles) {
-        const bool surr = h->mlp_deployed && h->mlp_have_params && h->mlp_have_norm;
+        const bool surr = surr_pass(h);
         // 32-bit addressing when every point slot and frame offset fits it
         const bool buf = buf_lines_ok(h->pitch) && h->fr_used < (size_t(1) << 30);
         PipeFn fn = surr ? (buf ? tk->fused_surr_buf : tk->fused_surr) : (buf ? tk->fused_buf : tk->fused);
@@ -982,9 +996,13 @@ int upload_points_d(IceHandle *h, int d, const float *soa, size_t host_pitch, ui
     }
     HIPCHK(h, hipSetDevice(h->device));
     h->nlines = (h->prm.flags & ICE_FLAG_COHESION_LINE) ? ICE_N_LINES_MAX : ICE_N_LINES;
-    if (n)
+    if (n) {
         HIPCHK(h, hipMemcpy2DAsync(h->dpts(d), h->pitch * sizeof(float), soa, host_pitch * sizeof(float),
                                    n * sizeof(float), h->nlines, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_fe_minus_identity, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->dpts(d),
+                           (size_t)h->pitch, n);
+        LAUNCHCHK(h, "k_fe_minus_identity");
+    }
     D.n = n;
     // the partition's disabled count starts at 0 (gpu_implementation5.cpp:137, gpu_partition.cpp:101-103)
     const int32_t zero = 0;
@@ -1068,6 +1086,12 @@ int download_d(IceHandle *h, int d, float *soa, size_t host_pitch, int n_lines, 
     if (soa) {
         uint32_t *u = reinterpret_cast<uint32_t *>(soa + (size_t)ICE_IDX_UTILITY * host_pitch);
         for (uint32_t i = 0; i < D.n; i++) u[i] &= ~kInternalBits;
+        // the device holds Fe - I; the reference's layout holds Fe
+        float *f00 = soa + (size_t)ICE_IDX_FE00 * host_pitch, *f11 = soa + (size_t)(ICE_IDX_FE00 + 3) * host_pitch;
+        for (uint32_t i = 0; i < D.n; i++) {
+            f00[i] += 1.f;
+            f11[i] += 1.f;
+        }
     }
     if (error_code) *error_code = h->host_cnt[d].err;
     if (disabled_count) *disabled_count = h->host_cnt[d].disabled;
@@ -1178,7 +1202,8 @@ int ice_destroy(IceHandle *h) {
                     (void *)h->s_dflag, (void *)h->s_drank, (void *)h->s_order, (void *)h->s_bsum,
                     (void *)h->d_desc, (void *)h->off_pre, (void *)h->d_tiles, (void *)h->fr[0], (void *)h->fr[1], (void *)h->mlp_p,
                     (void *)h->mlp_g, (void *)h->mlp_m, (void *)h->mlp_v, (void *)h->mlp_loss, (void *)h->mlp_pk, (void *)h->mlp_ipk, (void *)h->colors, (void *)h->tile_queue,
-                    (void *)h->mlp_fail, (void *)h->mlp_x, (void *)h->mlp_t})
+                    (void *)h->mlp_fail, (void *)h->mlp_x, (void *)h->mlp_t, (void *)h->mlp_rec,
+                    (void *)h->mlp_rec_cnt})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1783,6 +1808,9 @@ int mlp_pack(IceHandle *h, bool infer = true) {
     return ICE_OK;
 }
 
+int mlp_pairs_alloc(IceHandle *h, uint32_t n);
+int mlp_pairs_finish(IceHandle *h, uint32_t n, float *x, float *t, uint32_t *n_failed, bool ranges_from_data);
+
 int mlp_ready(IceHandle *h) {
     if (!h->mlp_have_params || !h->mlp_have_norm) {
         h->err = "surrogate parameters and standardisation not set";
@@ -1825,17 +1853,108 @@ int ice_mlp_generate(IceHandle *h, uint32_t n, uint64_t seed, const IceMlpSampli
         h->err = "bad sampling ranges";
         return ICE_E_ARG;
     }
+    if ((rc = mlp_pairs_alloc(h, n))) return rc;
+    const MlpSampling ms{S.logJ_min, S.logJ_max, S.D_min, S.D_max, S.jp_min, S.jp_max};
+    hipLaunchKernelGGL(k_mlp_pairs, dim3(std::min<unsigned>(blocks_for(n, 256), 16384)), dim3(256), 0, h->stream,
+                       mlp_phys(h), ms, seed, n, h->mlp_x, h->mlp_t, h->mlp_fail);
+    LAUNCHCHK(h, "k_mlp_pairs");
+    return mlp_pairs_finish(h, n, x, t, n_failed, false);
+}
+
+int ice_mlp_record(IceHandle *h, uint32_t capacity, float keep_fraction) {
+    if (!h || !(keep_fraction >= 0.f && keep_fraction <= 1.f)) return ICE_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graph(h);   // graphs hold the tile pass's arguments by value
+    if (h->mlp_rec) (void)hipFree(h->mlp_rec);
+    if (h->mlp_rec_cnt) (void)hipFree(h->mlp_rec_cnt);
+    h->mlp_rec = nullptr;
+    h->mlp_rec_cnt = nullptr;
+    h->mlp_rec_cap = 0;
+    if (capacity == 0) return ICE_OK;
+    HIPCHK(h, hipMalloc(&h->mlp_rec, sizeof(float) * kMlpIn * (size_t)capacity));
+    HIPCHK(h, hipMalloc(&h->mlp_rec_cnt, 2 * sizeof(unsigned long long)));
+    int rc = launch_fill_u32(h, reinterpret_cast<uint32_t *>(h->mlp_rec_cnt), 4, 0u);
+    if (rc) return rc;
+    h->mlp_rec_cap = capacity;
+    h->mlp_rec_keep = (uint32_t)std::min(4294967295.0, std::floor((double)keep_fraction * 4294967296.0));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ICE_OK;
+}
+
+int ice_mlp_record_keep(IceHandle *h, float keep_fraction) {
+    if (!h || !(keep_fraction >= 0.f && keep_fraction <= 1.f)) return ICE_E_ARG;
+    if (!h->mlp_rec) { h->err = "no recording (ice_mlp_record first)"; return ICE_E_STATE; }
+    const uint32_t keep = (uint32_t)std::min(4294967295.0, std::floor((double)keep_fraction * 4294967296.0));
+    if (keep == h->mlp_rec_keep) return ICE_OK;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graph(h);   // graphs hold the tile pass's arguments by value
+    h->mlp_rec_keep = keep;
+    return ICE_OK;
+}
+
+int ice_mlp_record_count(IceHandle *h, uint64_t *seen, uint32_t *stored) {
+    if (!h) return ICE_E_ARG;
+    unsigned long long c[2] = {0, 0};
+    if (h->mlp_rec_cnt) {
+        HIPCHK(h, hipSetDevice(h->device));
+        HIPCHK(h, hipMemcpyAsync(c, h->mlp_rec_cnt, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    if (seen) *seen = c[0];
+    if (stored) *stored = (uint32_t)std::min<unsigned long long>(c[1], h->mlp_rec_cap);
+    return ICE_OK;
+}
+
+int ice_mlp_records(IceHandle *h, float *x, uint32_t n_max, uint32_t *n) {
+    if (!h || (!x && n_max)) return ICE_E_ARG;
+    uint32_t stored = 0;
+    int rc = ice_mlp_record_count(h, nullptr, &stored);
+    if (rc) return rc;
+    const uint32_t k = std::min(stored, n_max);
+    if (k) {
+        HIPCHK(h, hipMemcpyAsync(x, h->mlp_rec, sizeof(float) * kMlpIn * (size_t)k, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    if (n) *n = k;
+    return ICE_OK;
+}
+
+int ice_mlp_generate_recorded(IceHandle *h, uint32_t n, uint64_t seed, float *x, float *t, uint32_t *n_failed) {
+    if (!h || n == 0) return ICE_E_ARG;
+    uint32_t stored = 0;
+    int rc = ice_mlp_record_count(h, nullptr, &stored);
+    if (rc) return rc;
+    if (stored == 0) { h->err = "no recorded trial states (ice_mlp_record, then run pipelined substeps)"; return ICE_E_STATE; }
+    if ((rc = mlp_alloc(h))) return rc;
+    if ((rc = mlp_pairs_alloc(h, n))) return rc;
+    hipLaunchKernelGGL(k_mlp_pairs_recorded, dim3(std::min<unsigned>(blocks_for(n, 256), 16384)), dim3(256), 0,
+                       h->stream, mlp_phys(h), (const float *)h->mlp_rec, stored, seed, n, h->mlp_x, h->mlp_t,
+                       h->mlp_fail);
+    LAUNCHCHK(h, "k_mlp_pairs_recorded");
+    return mlp_pairs_finish(h, n, x, t, n_failed, true);
+}
+
+}  // extern "C"
+
+namespace {
+
+// (Re)allocate the training pairs for n rows and clear the failure count.
+int mlp_pairs_alloc(IceHandle *h, uint32_t n) {
     if (h->mlp_x) (void)hipFree(h->mlp_x);
     if (h->mlp_t) (void)hipFree(h->mlp_t);
     h->mlp_x = h->mlp_t = nullptr;
     h->mlp_n = 0;
     HIPCHK(h, hipMalloc(&h->mlp_x, sizeof(float) * (size_t)n * kMlpIn));
     HIPCHK(h, hipMalloc(&h->mlp_t, sizeof(float) * (size_t)n * kMlpOut));
-    if ((rc = launch_fill_u32(h, h->mlp_fail, 1, 0u))) return rc;
-    const MlpSampling ms{S.logJ_min, S.logJ_max, S.D_min, S.D_max, S.jp_min, S.jp_max};
-    hipLaunchKernelGGL(k_mlp_pairs, dim3(std::min<unsigned>(blocks_for(n, 256), 16384)), dim3(256), 0, h->stream,
-                       mlp_phys(h), ms, seed, n, h->mlp_x, h->mlp_t, h->mlp_fail);
-    LAUNCHCHK(h, "k_mlp_pairs");
+    return launch_fill_u32(h, h->mlp_fail, 1, 0u);
+}
+
+// After a pairs kernel: the standardisation from (up to) the first 2^18
+// pairs, with ranges_from_data also the feature ranges the deployed network
+// accepts (the sampled ranges otherwise); copies to x, t when not NULL.
+int mlp_pairs_finish(IceHandle *h, uint32_t n, float *x, float *t, uint32_t *n_failed, bool ranges_from_data) {
     h->mlp_n = n;
     // standardisation from (up to) the first 2^18 pairs
     const uint32_t ns = std::min<uint32_t>(n, 1u << 18);
@@ -1859,12 +1978,26 @@ int ice_mlp_generate(IceHandle *h, uint32_t n, uint64_t seed, const IceMlpSampli
     for (int k = 0; k < kMlpIn; k++) stats(hx, kMlpIn, k, h->mlp_norm.xm[k], h->mlp_norm.xs[k]);
     for (int k = 0; k < kMlpOut; k++) stats(ht, kMlpOut, k, h->mlp_norm.ym[k], h->mlp_norm.ys[k]);
     h->mlp_have_norm = true;
+    if (ranges_from_data) {
+        // the span of the sampled features (log J, D, Jp_inv), to be accepted at deployment
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (uint32_t i = 0; i < ns; i++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], (double)hx[(size_t)i * kMlpIn + k]);
+                hi[k] = std::max(hi[k], (double)hx[(size_t)i * kMlpIn + k]);
+            }
+        h->mlp_samp = IceMlpSampling{lo[0], hi[0], std::max(lo[1], 1e-12), std::max(hi[1], 2e-12), lo[2], hi[2]};
+    }
     if (x) HIPCHK(h, hipMemcpy(x, h->mlp_x, sizeof(float) * (size_t)n * kMlpIn, hipMemcpyDeviceToHost));
     if (t) HIPCHK(h, hipMemcpy(t, h->mlp_t, sizeof(float) * (size_t)n * kMlpOut, hipMemcpyDeviceToHost));
     if (n_failed) *n_failed = failed;
     drop_graph(h);
     return ICE_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int ice_mlp_set_norm(IceHandle *h, const IceMlpNorm *norm) {
     if (!h || !norm) return ICE_E_ARG;

@@ -253,6 +253,14 @@ struct SurrArg {
     MlpNorm N;
     float tol;
     float lj0, lj1, d0, d1, jp0, jp1;   // feature ranges it was trained on
+    int infer;                          // 0: recording only, the exact map projects every point
+    // recording of the features of ridge-capped plastic trial states (rec_x
+    // null: off): a state is kept when a hash of (slot, step) falls below
+    // rec_keep (of 2^32), appended at rec_count[1] while it is below rec_cap;
+    // rec_count[0] counts every such state seen (64-bit)
+    float *rec_x;
+    unsigned long long *rec_count;
+    uint32_t rec_cap, rec_keep;
 };
 
 // BUF: 32-bit addressing of the points (BufLines) and frames, chosen by the
@@ -460,11 +468,25 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                             bool want = exact_rm && th == 1.f && s.p_tr >= rw.dpt &&
                                         s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, rw.dpt, P);
                             float xr[kMlpIn] = {0.f, 0.f, 1.f, 0.f};
-                            if (want) {
-                                mlp_features(vf, s.Jp, -rw.dpt * P.tan_phi, xr);
-                                want = xr[0] >= SA.lj0 && xr[0] <= SA.lj1 && xr[1] >= SA.d0 && xr[1] <= SA.d1 &&
-                                       xr[2] >= SA.jp0 && xr[2] <= SA.jp1;
+                            if (want) mlp_features(vf, s.Jp, -rw.dpt * P.tan_phi, xr);
+                            if (SA.rec_x) {   // uniform: the recording of trial states
+                                const uint32_t hsh = (uint32_t)mix64(((uint64_t)i << 24) ^
+                                                                     (uint64_t)__double_as_longlong(time));
+                                const bool keep = want && hsh < SA.rec_keep;
+                                const uint64_t bw = __ballot(want), bk = __ballot(keep);
+                                unsigned long long base = 0;
+                                if (lane == 0 && bw) {
+                                    atomicAdd(&SA.rec_count[0], (unsigned long long)__popcll(bw));
+                                    if (bk) base = atomicAdd(&SA.rec_count[1], (unsigned long long)__popcll(bk));
+                                }
+                                base = __shfl(base, 0, 64);
+                                const uint64_t slot_rec = base + __popcll(bk & ((1ull << lane) - 1ull));
+                                if (keep && slot_rec < SA.rec_cap)
+                                    for (int k = 0; k < kMlpIn; k++) SA.rec_x[slot_rec * kMlpIn + k] = xr[k];
                             }
+                            if (want)
+                                want = SA.infer && xr[0] >= SA.lj0 && xr[0] <= SA.lj1 && xr[1] >= SA.d0 &&
+                                       xr[1] <= SA.d1 && xr[2] >= SA.jp0 && xr[2] <= SA.jp1;
                             if (__any(want)) {
                                 float xs[kMlpIn];
 #pragma unroll
@@ -473,10 +495,10 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                                 mlp_infer(SA.M, xs, y);
                                 bool accepted = false;
                                 if (want) {
-                                    float s0, s1, jpf, pq, qq, viol;
-                                    accepted = mlp_decode(SA.N, y, s.Jp, rw.dpt, th, P, SA.tol, s0, s1, jpf, pq, qq, viol);
+                                    float m0, m1, jpf, pq, qq, viol;
+                                    accepted = mlp_decode(SA.N, y, xr[0], s.Jp, rw.dpt, th, P, SA.tol, m0, m1, jpf, pq, qq, viol);
                                     if (accepted) {
-                                        s.Fe = rebuild(vf, s0, s1);
+                                        s.Fe = rebuild(vf, m0, m1);
                                         s.Jp *= jpf;
                                         vf = invariants(s.Fe);
                                         if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;

@@ -66,6 +66,16 @@ __global__ void k_step_set(Counters *cnt, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) cnt->step = v;
 }
 
+// Fe -> Fe - I on the first n points of an uploaded domain (the device keeps
+// e = Fe - I in the Fe lines; download adds I back on the host).
+__global__ __launch_bounds__(256) void k_fe_minus_identity(float *__restrict__ pts, size_t pitch, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        pts[(size_t)ICE_IDX_FE00 * pitch + i] -= 1.f;
+        pts[(size_t)(ICE_IDX_FE00 + 3) * pitch + i] -= 1.f;
+    }
+}
+
 // partition_kernel_p2g (kernels.cuh:39-104)
 __global__ __launch_bounds__(256) void k_p2g_naive(DevParams P, Lines L, uint32_t n, float *__restrict__ gm,
                                                    float *__restrict__ gpx, float *__restrict__ gpy,

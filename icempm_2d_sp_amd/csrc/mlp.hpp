@@ -405,6 +405,25 @@ __global__ __launch_bounds__(256) void k_mlp_pairs(MlpPhys M, MlpSampling S, uin
     }
 }
 
+// Training pairs from recorded trial states: pair i takes record
+// hash(seed, i) mod n_rec (uniform, with replacement); its target is the
+// exact FP64 projection, as in k_mlp_pairs.  Records whose state the exact
+// map would not project with the ridge cap active count in n_failed.
+__global__ __launch_bounds__(256) void k_mlp_pairs_recorded(MlpPhys M, const float *__restrict__ R, uint32_t n_rec,
+                                                            uint64_t seed, uint32_t n, float *__restrict__ X,
+                                                            float *__restrict__ T, uint32_t *__restrict__ n_failed) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(mix64(seed ^ mix64((uint64_t)i)) % n_rec);
+        double x[kMlpIn], y[kMlpOut] = {0, 0, 0};
+        for (int k = 0; k < kMlpIn; k++) x[k] = (double)R[(size_t)r * kMlpIn + k];
+        const bool ok = exact_projection_d(x[0], x[1], x[2], 1e5 * exp(x[3]), M, true, y) && y[0] < 0.0 && y[1] > 0.0;
+        if (!ok) atomicAdd(n_failed, 1u);
+        for (int k = 0; k < kMlpIn; k++) X[(size_t)i * kMlpIn + k] = (float)x[k];
+        const double t[kMlpOut] = {ok ? log(-y[0]) : 0.0, ok ? log(y[1]) : 0.0, ok ? y[2] : 0.0};
+        for (int k = 0; k < kMlpOut; k++) T[(size_t)i * kMlpOut + k] = (float)t[k];
+    }
+}
+
 // One forward + backward pass over `batch` samples; adds the gradient of the
 // mean squared error (standardised targets, mean over samples and outputs)
 // into grad and the loss into *loss.  Sample s is row s (seq != 0) or a hashed
@@ -592,25 +611,38 @@ __global__ __launch_bounds__(64) void k_mlp_infer(MlpDev M, MlpNorm N, const flo
 }
 
 // The surrogate's answer for one point: raw outputs from standardised ones,
-// the projected state's (p', q') and its yield violation, and whether it is
-// accepted at tolerance tol (else the exact map runs).  s0, s1 are the new
-// singular values, jpf the factor on Jp_inv.
-__device__ __forceinline__ bool mlp_decode(const MlpNorm &N, const float y[kMlpOut], float jp, float dpt, float th,
-                                           const DevParams &P, float tol, float &s0, float &s1, float &jpf,
-                                           float &p_out, float &q_out, float &viol) {
+// the projected state's (p', q'), the yield violation of the projection, and
+// whether it is accepted at tolerance tol (else the exact map runs).  m0, m1
+// are the new singular values minus one, jpf the factor on Jp_inv.
+//
+// The exact map (kernels.cuh:441-466) picks p1 = min(p_tr, p_ridge) and
+// q1 = q_yield(p1), then builds the new singular values from
+// Je_new = sqrt(1 - 2 p1 / kappa) and t = q1 J_tr / (2 mu) as
+// s0'^2, s1'^2 = Je_new +- t; the new state itself lies off the yield
+// surface by O(t^2) and by J_tr / J'.  From the network's log J' and
+// D' = log(s0' / s1'): Je_new = J' cosh D' and t = J' sinh D', hence
+// p1 = -kappa/2 (J'^2 cosh^2 D' - 1) and q1 = 2 mu J' sinh D' / J_tr, and the
+// violation |q1 - q_yield(p1)| / q_yield(p1) is zero for the exact map.
+__device__ __forceinline__ bool mlp_decode(const MlpNorm &N, const float y[kMlpOut], float logJ_tr, float jp,
+                                           float dpt, float th, const DevParams &P, float tol, float &m0, float &m1,
+                                           float &jpf, float &p_out, float &q_out, float &viol) {
     const float lj = -expf(N.ym[0] + N.ys[0] * y[0]);
     const float D = expf(N.ym[1] + N.ys[1] * y[1]);
     const float dj = N.ym[2] + N.ys[2] * y[2];
     p_out = -P.half_kappa * expm1f(2.f * lj);
     q_out = 2.f * P.mu * sinhf(D);
-    const float qy = q_yield_at(p_out, dpt, P);
-    viol = fabsf(q_out - qy) / fmaxf(qy, 1.f);
+    // log cosh D' (D'^2 / 2 - D'^4 / 12 below 0.1, where coshf would round to 1)
+    const float lc = D < 0.1f ? 0.5f * D * D * (1.f - D * D * (1.f / 6.f)) : logf(coshf(D));
+    const float p1 = -P.half_kappa * expm1f(2.f * (lj + lc));
+    const float q1 = 2.f * P.mu * expf(lj - logJ_tr) * sinhf(D);
+    const float qy = q_yield_at(p1, dpt, P);
+    viol = fabsf(q1 - qy) / fmaxf(qy, 1.f);
     const float p_ridge = P.ridge * th * jp;
-    s0 = expf(0.5f * (lj + D));
-    s1 = expf(0.5f * (lj - D));
+    m0 = expm1f(0.5f * (lj + D));   // new singular values minus one
+    m1 = expm1f(0.5f * (lj - D));
     jpf = expf(dj);
-    return viol <= tol && p_out <= p_ridge * (1.f + tol) + 1.f && D >= 0.f && __builtin_isfinite(s0) &&
-           __builtin_isfinite(s1) && __builtin_isfinite(jpf);
+    return viol <= tol && p1 <= p_ridge * (1.f + tol) + 1.f && D >= 0.f && __builtin_isfinite(m0) &&
+           __builtin_isfinite(m1) && __builtin_isfinite(jpf);
 }
 
 // Surrogate vs exact FP64 for n raw trial states (logJ, D, Jp, log(c/1e5)):
@@ -629,8 +661,8 @@ __global__ __launch_bounds__(64) void k_mlp_eval(MlpDev M, MlpNorm N, DevParams 
     if (s >= n) return;
     const float coh = 1e5f * expf(xr[3]);
     const float dpt = -coh * P.inv_tan_phi;
-    float s0, s1, jpf, ps, qs, viol;
-    const bool ok = mlp_decode(N, y, xr[2], dpt, 1.f, P, tol, s0, s1, jpf, ps, qs, viol);
+    float m0, m1, jpf, ps, qs, viol;
+    const bool ok = mlp_decode(N, y, xr[0], xr[2], dpt, 1.f, P, tol, m0, m1, jpf, ps, qs, viol);
     double ye[kMlpOut];
     double pe = 0.0, qe = 0.0;
     if (exact_projection_d(xr[0], xr[1], xr[2], (double)coh, Mp, false, ye)) {

@@ -189,7 +189,7 @@ struct G2PState {
     uint32_t u;
     int cx, cy;
     float x, y;
-    M2 Fe;
+    M2 Fe;          // Fe - I, as the device stores it
     float Jp;
     float p_tr, q_tr;
     bool cell_updated;
@@ -223,13 +223,13 @@ __device__ __forceinline__ void g2p_predict(const DevParams &P, G2PState &s, flo
         s.newly_disabled = true;
     }
     if (s.x > 0.5f || s.x < -0.5f || s.y > 0.5f || s.y < -0.5f) err |= ICE_ERR_POINT_JUMP_CELLS;
-    // Fe = (I + dt C) Fe
-    const M2 F = s.Fe;
+    // Fe = (I + dt C) Fe on e = Fe - I: e' = e + dt C + dt C e
+    const M2 e = s.Fe;
     const float c00 = P.dt * C.a00, c01 = P.dt * C.a01, c10 = P.dt * C.a10, c11 = P.dt * C.a11;
-    s.Fe.a00 = F.a00 + (c00 * F.a00 + c01 * F.a10);
-    s.Fe.a01 = F.a01 + (c00 * F.a01 + c01 * F.a11);
-    s.Fe.a10 = F.a10 + (c10 * F.a00 + c11 * F.a10);
-    s.Fe.a11 = F.a11 + (c10 * F.a01 + c11 * F.a11);
+    s.Fe.a00 = e.a00 + (c00 + (c00 * e.a00 + c01 * e.a10));
+    s.Fe.a01 = e.a01 + (c01 + (c00 * e.a01 + c01 * e.a11));
+    s.Fe.a10 = e.a10 + (c10 + (c10 * e.a00 + c11 * e.a10));
+    s.Fe.a11 = e.a11 + (c11 + (c10 * e.a01 + c11 * e.a11));
     if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
     vf = invariants(s.Fe);
     compute_pq(vf, P, s.p_tr, s.q_tr);

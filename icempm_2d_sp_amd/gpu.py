@@ -304,6 +304,72 @@ class SurrogateMixin:
                                             abi.ptr(x), abi.ptr(t), C.byref(failed)), "ice_mlp_generate")
         return x, t, failed.value
 
+    def mlp_record(self, capacity: int, keep_fraction: float = 1.0):
+        """Record ridge-capped plastic trial states of the following substeps
+        (capacity 0 stops and frees)."""
+        self._chk(self.lib.ice_mlp_record(self.h, capacity, keep_fraction), "ice_mlp_record")
+
+    def mlp_record_keep(self, keep_fraction: float):
+        """Change the kept fraction of the open recording."""
+        self._chk(self.lib.ice_mlp_record_keep(self.h, keep_fraction), "ice_mlp_record_keep")
+
+    def mlp_record_count(self):
+        """(states seen, rows stored)."""
+        seen, stored = C.c_uint64(0), C.c_uint32(0)
+        self._chk(self.lib.ice_mlp_record_count(self.h, C.byref(seen), C.byref(stored)), "ice_mlp_record_count")
+        return int(seen.value), int(stored.value)
+
+    def mlp_records(self, n_max: int) -> np.ndarray:
+        """The stored trial-state rows [n][4] (log J, D, Jp_inv, log(c / 1e5))."""
+        x = np.zeros((max(n_max, 1), 4), np.float32)
+        n = C.c_uint32(0)
+        self._chk(self.lib.ice_mlp_records(self.h, abi.ptr(x), n_max, C.byref(n)), "ice_mlp_records")
+        return x[:n.value]
+
+    def train_on_recorded(self, first_step: int, record_steps: int = 16, pairs: int = 10_000_000,
+                          train_steps=(8000, 6000, 6000), lrs=(2e-3, 5e-4, 1e-4), batch: int = 8192):
+        """Train the learned projection on the handle's own workload: record
+        about `pairs` ridge-capped plastic trial states (hash-sampled) over
+        `record_steps` substeps, draw `pairs` training pairs from the records
+        (targets from the exact FP64 map) and train with Adam.  The first
+        substep only counts the states; before each later one the kept
+        fraction is set from the last substep's count so that the records
+        spread over the whole window even where the count grows as the run
+        develops.  Runs 1 + record_steps substeps from first_step.  Returns a
+        report dict (states seen, kept fractions, loss, seconds)."""
+        import time
+        cap, target = int(1.2 * pairs) + 1024, 1.1 * pairs
+        self.mlp_record(cap, 0.0)                   # the first substep counts only
+        keeps, seen_prev, seen1 = [], 0, None
+        for k in range(record_steps + 1):
+            self.run(first_step + k, 1, graph=False)
+            seen, stored = self.mlp_record_count()
+            last, seen_prev = seen - seen_prev, seen
+            seen1 = last if seen1 is None else seen1
+            rem = record_steps - k
+            if rem:
+                keeps.append(min(1.0, max(target - stored, 0.0) / max(last * rem, 1)))
+                self.mlp_record_keep(keeps[-1])
+        seen, stored = self.mlp_record_count()
+        t0 = time.perf_counter()
+        _, _, failed = self.mlp_generate_recorded(pairs, seed=3)
+        self.mlp_record(0, 0.0)
+        self.mlp_init(seed=21)
+        loss = np.concatenate([self.mlp_train(st, batch=batch, lr=lr, seed=5 + k)
+                               for k, (st, lr) in enumerate(zip(train_steps, lrs))])
+        return {"states_seen_per_substep": seen1, "states_seen": seen, "states_stored": stored,
+                "keep_fractions": [round(k, 4) for k in keeps], "pairs": pairs, "pairs_failed": failed, "train_steps": int(loss.shape[0]),
+                "seconds": time.perf_counter() - t0, "loss_first": float(loss[0]),
+                "loss_last": float(loss[-100:].mean())}
+
+    def mlp_generate_recorded(self, n: int, seed: int = 1, copy: bool = False):
+        x = np.zeros((n, 4), np.float32) if copy else None
+        t = np.zeros((n, 3), np.float32) if copy else None
+        failed = C.c_uint32(0)
+        self._chk(self.lib.ice_mlp_generate_recorded(self.h, n, seed, abi.ptr(x), abi.ptr(t), C.byref(failed)),
+                  "ice_mlp_generate_recorded")
+        return x, t, failed.value
+
     def mlp_norm(self) -> "abi.IceMlpNorm":
         nrm = abi.IceMlpNorm()
         self._chk(self.lib.ice_mlp_get_norm(self.h, C.byref(nrm)), "ice_mlp_get_norm")
@@ -391,6 +457,12 @@ class Simulation(SurrogateMixin):
 
     def _chk(self, rc, what):
         abi.check(rc, self.h, what)
+
+    def reset_points(self):
+        """Upload the scene's initial points again (the state restarts)."""
+        soa = np.ascontiguousarray(self.scene.soa, np.float32)
+        self._chk(self.lib.ice_upload_points(self.h, abi.ptr(soa), soa.shape[1], soa.shape[1], soa.shape[0]),
+                  "ice_upload_points")
 
     def close(self):
         if self.h:
@@ -514,6 +586,10 @@ class BatchSimulation(SurrogateMixin):
         for name, value in (options or {}).items():
             self._chk(self.lib.ice_set_option(self.h, name.encode(), int(value)), f"ice_set_option({name})")
         self.nlines = self.scenes[0].nlines
+        self.reset_points()
+
+    def reset_points(self):
+        """Upload every domain's initial points, status and forcing (the state restarts)."""
         for d, s in enumerate(self.scenes):
             soa = np.ascontiguousarray(s.soa, np.float32)
             st = np.ascontiguousarray(s.status, np.uint8)

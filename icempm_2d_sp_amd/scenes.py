@@ -264,18 +264,30 @@ def batch64_compressed(seed: int = 5005, n_domains: int = 64, min_side: int = 12
     plastically onto the ridge-capped yield surface every substep (the trial
     strain increment dt * strain_rate exceeds the yield strain)."""
     doms = batch64_uneven(seed=seed, n_domains=n_domains, min_side=min_side, max_side=max_side)
-    out = []
-    for sc in doms:
-        soa = sc.soa
-        gx, gy = sc.params.GridXTotal, sc.params.GridYTotal
-        xi = (u32_line(soa, abi.IDX_CELL) & 0xFFFF).astype(np.float64) + soa[abi.IDX_POSX]
-        x_wall = gx - 2.0
-        soa[abi.IDX_VELX] = (strain_rate * (x_wall - xi) * sc.params.cellsize).astype(np.float32)
-        soa[abi.IDX_VELX + 1] = 0.0
-        soa[abi.IDX_BP00] = -strain_rate
-        soa[abi.IDX_BP00 + 1:abi.IDX_BP00 + 4] = 0.0
-        N = gx * gy
-        wind = np.zeros((2, N), np.float32)
-        wind[0] = wind_pa
-        out.append(Scene(sc.params, soa, sc.status, None, wind, sc.name + "c"))
-    return out
+    return [compressed(sc, strain_rate, wind_pa) for sc in doms]
+
+
+def compressed(sc: Scene, strain_rate: float = 1.0, wind_pa: float = 1.0, band: float | None = None) -> Scene:
+    """The scene's ice in uniaxial compression against the fixed wall at x_max
+    (the first wall node column, x_w = gx - 2): v_x = strain_rate (x_w - x)
+    (m/s, Bp = grad v), v_y = 0, under a uniform wind stress of wind_pa along
+    +x (replacing the scene's forcing), so that points project plastically
+    onto the ridge-capped yield surface every substep.  With `band` (cells),
+    the compression acts within that distance of the wall and the ice beyond
+    moves at the band's edge speed, strain_rate * band * h (Bp = 0 there), so
+    that no velocity reaches the node clamp (sqrt(0.1) h / (4 dt))."""
+    soa = sc.soa
+    gx, gy = sc.params.GridXTotal, sc.params.GridYTotal
+    xi = (u32_line(soa, abi.IDX_CELL) & 0xFFFF).astype(np.float64) + soa[abi.IDX_POSX]
+    x_wall = gx - 2.0
+    dist = x_wall - xi
+    inside = np.ones(dist.shape, bool) if band is None else dist < band
+    if band is not None:
+        dist = np.minimum(dist, band)
+    soa[abi.IDX_VELX] = (strain_rate * dist * sc.params.cellsize).astype(np.float32)
+    soa[abi.IDX_VELX + 1] = 0.0
+    soa[abi.IDX_BP00] = np.where(inside, -strain_rate, 0.0).astype(np.float32)
+    soa[abi.IDX_BP00 + 1:abi.IDX_BP00 + 4] = 0.0
+    wind = np.zeros((2, gx * gy), np.float32)
+    wind[0] = wind_pa
+    return Scene(sc.params, soa, sc.status, None, wind, sc.name + "c")

@@ -301,6 +301,25 @@ int ice_mlp_default_sampling(IceMlpSampling *s);
  * *n_failed = pairs whose sampling found no plastic state. */
 int ice_mlp_generate(IceHandle *h, uint32_t n, uint64_t seed, const IceMlpSampling *s, float *x, float *t,
                      uint32_t *n_failed);
+/* Record the trial states the exact map projects on the ridge-capped
+ * plastic branch (thickness 1) during the following pipelined substeps, as
+ * feature rows (log J, D, Jp_inv, log(c / 1e5)): each state is kept with
+ * probability keep_fraction (a hash of its slot and step), up to `capacity`
+ * rows.  capacity 0 stops recording and frees the rows.  Recording runs the
+ * tile pass of the learned projection (the network itself only when it is
+ * deployed). */
+int ice_mlp_record(IceHandle *h, uint32_t capacity, float keep_fraction);
+/* Change the kept fraction of an open recording; rows stored so far stay. */
+int ice_mlp_record_keep(IceHandle *h, float keep_fraction);
+/* States seen on that branch since ice_mlp_record, and rows stored. */
+int ice_mlp_record_count(IceHandle *h, uint64_t *seen, uint32_t *stored);
+/* Copy up to n_max stored rows [n][4] to x; *n = rows copied. */
+int ice_mlp_records(IceHandle *h, float *x, uint32_t n_max, uint32_t *n);
+/* As ice_mlp_generate, from the recorded states: n pairs, row i from a
+ * hash-chosen stored state, its target from the exact projection in FP64.
+ * Also sets the feature ranges the deployed network accepts to the span of
+ * the pairs. */
+int ice_mlp_generate_recorded(IceHandle *h, uint32_t n, uint64_t seed, float *x, float *t, uint32_t *n_failed);
 int ice_mlp_set_norm(IceHandle *h, const IceMlpNorm *norm);
 int ice_mlp_get_norm(IceHandle *h, IceMlpNorm *norm);
 /* FP32 master parameters (ICE_MLP_PARAMS). ice_mlp_init_params draws
@@ -318,9 +337,12 @@ int ice_mlp_forward(IceHandle *h, const float *x, uint32_t n, float *y);
 /* Gradient of the mean squared error over the n rows (x raw, t raw) into
  * grad (ICE_MLP_PARAMS), loss into *loss; parameters unchanged. */
 int ice_mlp_grad(IceHandle *h, const float *x, const float *t, uint32_t n, float *grad, float *loss);
-/* Per raw trial state x [n][4]: out [n][6] = surrogate (p', q'), exact FP64
- * (p', q'), yield violation |q' - q_yield(p')| / q_yield(p') of the
- * surrogate's state, 1 if the surrogate is accepted at tolerance `tol`. */
+/* Per raw trial state x [n][4]: out [n][6] = surrogate (p', q') of the new
+ * state, exact FP64 (p', q'), yield violation |q1 - q_yield(p1)| / q_yield(p1)
+ * of the projection the surrogate implies (p1 = -kappa/2 (J'^2 cosh^2 D' - 1),
+ * q1 = 2 mu J' sinh D' / J_trial: the point the exact map places on the yield
+ * surface, kernels.cuh:441-466, zero for the exact map), 1 if the surrogate
+ * is accepted at tolerance `tol`. */
 int ice_mlp_eval(IceHandle *h, const float *x, uint32_t n, float tol, float *out);
 /* No counterpart in the reference.  ice_mlp_eval for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
