@@ -139,6 +139,8 @@ def main() -> int:
                          "frame: the per-frame visualisation arrays of the 4096x2048 domain; "
                          "compressed: the plain batched substep of the surrogate workload's domains (fast flow)")
     ap.add_argument("--surrogate-tol", type=float, default=1e-4, help="yield-violation tolerance of the fallback")
+    ap.add_argument("--single-handle", action="store_true",
+                    help="surrogate workload: the surrogate leg only, one handle, eager launches (for profiling)")
     ap.add_argument("--copy-only", type=float, default=0.0,
                     help="only time a plain copy of this many GB (HBM reference / counter calibration)")
     ap.add_argument("--no-secondary", action="store_true",
@@ -304,13 +306,13 @@ def main() -> int:
     return 0
 
 
-def _timed_run(sim, steps: int) -> float:
+def _timed_run(sim, steps: int, graph: bool = True) -> float:
     """One graph chunk of warm-up (capture, instantiation), then `steps`
-    substeps of graph replays, timed."""
-    sim.run(0, 16, graph=True)
+    substeps of graph replays, timed (graph False: eager launches)."""
+    sim.run(0, 16, graph=graph)
     sim.sync()
     t0 = time.perf_counter()
-    sim.run(16, steps, graph=True)
+    sim.run(16, steps, graph=graph)
     sim.sync()
     return time.perf_counter() - t0
 
@@ -400,7 +402,7 @@ def surrogate_accuracy(sim, x, tol: float) -> dict:
             "heldout_states": int(x.shape[0])}
 
 
-def measure_surrogate(steps: int, tol: float, naive_leg: bool = False) -> dict:
+def measure_surrogate(steps: int, tol: float, naive_leg: bool = False, single_handle: bool = False) -> dict:
     """batch64_compressed: 64 uneven domains in uniaxial compression at 1/s
     against the fixed wall at x_max under 1 Pa wind (most points project
     plastically every substep).  Times (a) plain batched substeps (exact
@@ -408,7 +410,8 @@ def measure_surrogate(steps: int, tol: float, naive_leg: bool = False) -> dict:
     the exact fallback at `tol`, (c) the plain single-domain runs one after
     another, and with naive_leg (d) the reference's launch structure one
     domain after another.  The network is trained on the GPU first (not
-    timed)."""
+    timed).  single_handle: (b) only, with eager launches, for profiling its
+    kernels (one handle in the process and no HIP graphs)."""
     import numpy as np
     from icempm_2d_sp_amd import abi, scenes
     from icempm_2d_sp_amd.gpu import BatchSimulation, Simulation
@@ -416,14 +419,16 @@ def measure_surrogate(steps: int, tol: float, naive_leg: bool = False) -> dict:
     steps = max(16, (steps // 16) * 16)
     print(f"[bench] {len(doms)} domains, {sum(d.n for d in doms)} points; {steps} substeps", file=sys.stderr,
           flush=True)
-    plain = BatchSimulation(doms)
-    n_active = plain.n_points()
-    wall_plain = _timed_run(plain, steps)
-    pa, _, _, _ = plain.download(0)
-    plain.close()
-    print(f"[bench] plain batched {wall_plain:.3f}s", file=sys.stderr, flush=True)
+    pa = wall_plain = None
+    if not single_handle:
+        plain = BatchSimulation(doms)
+        wall_plain = _timed_run(plain, steps)
+        pa, _, _, _ = plain.download(0)
+        plain.close()
+        print(f"[bench] plain batched {wall_plain:.3f}s", file=sys.stderr, flush=True)
 
     surr = BatchSimulation(doms)
+    n_active = surr.n_points()
     # the network is trained on the trial states of this workload's first 17
     # substeps (the 256x128 floe's run has none: it never yields), and judged
     # on those of the 4 substeps after them
@@ -436,39 +441,44 @@ def measure_surrogate(steps: int, tol: float, naive_leg: bool = False) -> dict:
     surr.reset_points()
     surr.mlp_deploy(True, tol)
     c0 = surr.mlp_counts()
-    wall_surr = _timed_run(surr, steps)
+    wall_surr = _timed_run(surr, steps, graph=not single_handle)
     c1 = surr.mlp_counts()
     n_acc, n_fb = c1[0] - c0[0], c1[1] - c0[1]   # warm-up chunk included: counts per substep below
     pb, _, err_surr, _ = surr.download(0)
     surr.close()
     print(f"[bench] surrogate batched {wall_surr:.3f}s, projections {n_acc}, fallbacks {n_fb}", file=sys.stderr,
           flush=True)
-    ka = np.argsort(pa[abi.IDX_POINT].view(np.uint32), kind="stable")
-    kb = np.argsort(pb[abi.IDX_POINT].view(np.uint32), kind="stable")
-    dfe = float(np.abs(pa[abi.IDX_FE00:abi.IDX_FE00 + 4][:, ka].astype(np.float64) -
-                       pb[abi.IDX_FE00:abi.IDX_FE00 + 4][:, kb]).max())
-    wall_seq = 0.0
-    for sc in doms:
-        s1 = Simulation(sc)
-        wall_seq += _timed_run(s1, steps)
-        s1.close()
-    print(f"[bench] plain sequential {wall_seq:.3f}s", file=sys.stderr, flush=True)
+    dfe = wall_seq = None
+    if not single_handle:
+        ka = np.argsort(pa[abi.IDX_POINT].view(np.uint32), kind="stable")
+        kb = np.argsort(pb[abi.IDX_POINT].view(np.uint32), kind="stable")
+        dfe = float(np.abs(pa[abi.IDX_FE00:abi.IDX_FE00 + 4][:, ka].astype(np.float64) -
+                           pb[abi.IDX_FE00:abi.IDX_FE00 + 4][:, kb]).max())
+        wall_seq = 0.0
+        for sc in doms:
+            s1 = Simulation(sc)
+            wall_seq += _timed_run(s1, steps)
+            s1.close()
+        print(f"[bench] plain sequential {wall_seq:.3f}s", file=sys.stderr, flush=True)
+
+    def rate(wall):
+        return None if wall is None else {"value": n_active * steps / wall, "ms_per_step": wall * 1000 / steps}
     out = {
         "workload": "batch64_compressed: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, uniaxial "
                     "compression at 1/s against the fixed wall at x_max, 1.0 Pa wind along +x, 4 points/cell",
         "points": n_active, "steps": steps, "surrogate_tol": tol, "dtype": "f32 state, bf16 MLP",
         "value": n_active * steps / wall_surr, "unit": "particle-updates/s", "ms_per_step": wall_surr * 1000 / steps,
-        "plain_batched": {"value": n_active * steps / wall_plain, "ms_per_step": wall_plain * 1000 / steps},
-        "plain_sequential": {"value": n_active * steps / wall_seq, "ms_per_step": wall_seq * 1000 / steps},
-        "surrogate_batched_over_plain_sequential": wall_seq / wall_surr,
-        "surrogate_batched_over_plain_batched": wall_plain / wall_surr,
+        "plain_batched": rate(wall_plain),
+        "plain_sequential": rate(wall_seq),
+        "surrogate_batched_over_plain_sequential": None if wall_seq is None else wall_seq / wall_surr,
+        "surrogate_batched_over_plain_batched": None if wall_plain is None else wall_plain / wall_surr,
         "ridge_capped_plastic_fraction_per_substep": (n_acc + n_fb) / (n_active * (steps + 16)),
         "surrogate_accepted_fraction": n_acc / max(n_acc + n_fb, 1),
         "training": train,
         "stress_error_vs_fp64": acc,
         "final_Fe_max_abs_diff_first_domain": dfe, "error_word": err_surr,
     }
-    if naive_leg:
+    if naive_leg and not single_handle:
         # the reference's own launch structure (unsorted points, global atomics,
         # ICE_FLAG_NAIVE, eager launches), one domain after another
         naive_steps = 64
@@ -495,7 +505,7 @@ def measure_surrogate(steps: int, tol: float, naive_leg: bool = False) -> dict:
 def bench_surrogate(args) -> int:
     """batch64_compressed (measure_surrogate, with the naive leg); value =
     the surrogate batched rate."""
-    r = measure_surrogate(args.steps, args.surrogate_tol, naive_leg=True)
+    r = measure_surrogate(args.steps, args.surrogate_tol, naive_leg=True, single_handle=args.single_handle)
     out = {
         "metric": "particle-updates/s per substep, 64 batched uneven domains, surrogate + batched vs plain",
         "value": r["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": r["steps"], "warmup": 16,

@@ -195,6 +195,10 @@ __device__ __forceinline__ void frame_scatter_global(const P2GPoint &q, const Do
 #ifndef ICE_PIPE_PASSES
 #define ICE_PIPE_PASSES 4
 #endif
+// 1: a batch's point loads are issued while the previous batch is summed.
+#ifndef ICE_PIPE_PREFETCH
+#define ICE_PIPE_PREFETCH 1
+#endif
 constexpr int kPasses = ICE_PIPE_PASSES;
 constexpr int kStage = kPasses * kTileThreads;
 constexpr int kStageFields = 9;   // x, y, mass, mvx, mvy, s00, s01, s10, s11 (+ the cell, 16 bits)
@@ -203,10 +207,20 @@ constexpr int kStageFields = 9;   // x, y, mass, mvx, mvy, s00, s01, s10, s11 (+
 static_assert(kStageFields * kStage >= 27 * kTileThreads, "staging area too small for the cell stencils");
 
 // What G2P reads of a point, loaded ahead of the computation.
+// A point's G2P inputs as loaded.  coh is the cohesion line, or (no
+// cohesion line) the thickness line: one unconditional load from a uniformly
+// chosen line, turned into the Drucker-Prager threshold only where it is
+// used, so that no computation right after the loads makes the wave wait for
+// them (the passes' loads stay in flight together).
 struct G2PRaw {
     uint32_t u, cell;
-    float x, y, f00, f01, f10, f11, jp, th, dpt;
+    float x, y, f00, f01, f10, f11, jp, th, coh;
 };
+
+// dp_threshold from the raw line value
+__device__ __forceinline__ float raw_dpt(const DevParams &P, float coh) {
+    return P.has_cohesion ? -coh * P.inv_tan_phi : P.DP_threshold_p;
+}
 
 template <class LT>
 __device__ __forceinline__ void load_g2p_raw(const LT &L, const DevParams &P, uint32_t i, G2PRaw &r) {
@@ -220,7 +234,7 @@ __device__ __forceinline__ void load_g2p_raw(const LT &L, const DevParams &P, ui
     r.f11 = L.get(ICE_IDX_FE00 + 3, i);
     r.jp = L.get(ICE_IDX_JP_INV, i);
     r.th = L.get(ICE_IDX_THICKNESS, i);
-    r.dpt = dp_threshold(P, L, i);
+    r.coh = L.get(P.has_cohesion ? ICE_IDX_COHESION : ICE_IDX_THICKNESS, i);
 }
 
 // A point's P2G terms into staging slot jj (mass 0 marks an empty slot), and
@@ -266,8 +280,16 @@ struct SurrArg {
 // BUF: 32-bit addressing of the points (BufLines) and frames, chosen by the
 // host when the point buffer and the frames are small enough (buf_lines_ok,
 // frames below 2^30 floats).
+// Workgroups per CU the register budget is sized for.  Tiles up to 512 cells
+// leave room in LDS for 3 (<= 168 VGPRs); the surrogate variant is held to
+// that budget (its network would otherwise take it past 168 and to 2 waves
+// per SIMD, ~30 % slower), the plain one is not (it stays below 168 by itself
+// and schedules better without the bound).
+template <int TW, int TH, bool SURR>
+constexpr int tile_min_blocks() { return SURR && TW * TH <= 512 ? 3 : 2; }
+
 template <int TW, int TH, bool FUSED, bool SURR = false, bool PERSIST = false, bool BUF = false>
-__global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0, StepArg S, SurrArg SA,
+__global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) void k_tile_pipeline(DevParams P0, StepArg S, SurrArg SA,
                                                                 const DomainDesc *__restrict__ D,
                                                                 const TileRef *__restrict__ T, int ntiles, Lines L0,
                                                                 const uint32_t *__restrict__ cs0, GridLines G0,
@@ -328,6 +350,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
         double time = 0.0;
         bool rec = false;
         float ramp = 1.f;
+
         if constexpr (FUSED) {
             resolve_step(S, P, time, rec);
             ramp = current_ramp(time);
@@ -388,27 +411,50 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                 f0 = s_pre[cc] + (b - s_rb[cc]);
                 flen = cs[cell + 1] - b;
             }
-            for (uint32_t sub = 0; sub < n_round; sub += kStage) {   // uniform bound
-                // locate this lane's point of each pass (slot, or ~0u for none)
-                uint32_t slot[kPasses];
+            // this lane's point of each pass of the batch at `base` (slot, or ~0u for none)
+            auto locate = [&](uint32_t base, uint32_t (&sl)[kPasses]) {
 #pragma unroll
                 for (int p = 0; p < kPasses; p++) {
-                    slot[p] = ~0u;
-                    const uint32_t jw = sub + (uint32_t)(p * kTileThreads + wave * 64);
+                    sl[p] = ~0u;
+                    const uint32_t jw = base + (uint32_t)(p * kTileThreads + wave * 64);
                     if (jw < n_round) {   // wave-uniform
                         const int cb = flat_col_n(jw, s_pre, ncol);
                         const uint32_t j = jw + lane;
                         if (j < n_round) {
                             const int c = col_from_n(j, cb, s_pre, ncol);
-                            slot[p] = s_rb[c] + (j - s_pre[c]);
+                            sl[p] = s_rb[c] + (j - s_pre[c]);
                         }
                     }
+                }
+            };
+            // PF: the point loads of the next batch are issued right after the
+            // staging barrier, so that their latency passes under the batch's
+            // three LDS-only P2G phases (barriers wait on LDS counts only)
+            constexpr bool PF = FUSED && !SURR && ICE_PIPE_PREFETCH;
+            uint32_t slot_n[kPasses];
+            G2PRaw raw_n[PF ? kPasses : 1];
+            if constexpr (PF) {
+                locate(0u, slot_n);
+#pragma unroll
+                for (int p = 0; p < kPasses; p++)
+                    if (slot_n[p] != ~0u) load_g2p_raw(L, P, slot_n[p], raw_n[p]);
+            }
+            for (uint32_t sub = 0; sub < n_round; sub += kStage) {   // uniform bound
+                uint32_t slot[kPasses];
+                if constexpr (PF) {
+#pragma unroll
+                    for (int p = 0; p < kPasses; p++) slot[p] = slot_n[p];
+                } else {
+                    locate(sub, slot);
                 }
                 if constexpr (FUSED) {
                     // every pass's loads first, then the computation (the surrogate
                     // variant, short of registers, loads each pass when it computes it)
                     G2PRaw raw[kPasses];
-                    if constexpr (!SURR) {
+                    if constexpr (PF) {
+#pragma unroll
+                        for (int p = 0; p < kPasses; p++) raw[p] = raw_n[p];
+                    } else if constexpr (!SURR) {
 #pragma unroll
                         for (int p = 0; p < kPasses; p++)
                             if (slot[p] != ~0u) load_g2p_raw(L, P, slot[p], raw[p]);
@@ -435,40 +481,40 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                         s.Fe = M2{rw.f00, rw.f01, rw.f10, rw.f11};
                         s.Jp = rw.jp;
                         const float th = rw.th;
+                        const float dpt = raw_dpt(P, rw.coh);
                         float pvx = 0.f, pvy = 0.f;
                         M2 C{0.f, 0.f, 0.f, 0.f};
                         FInv vf;
+                        // outside the frame cannot happen while the drift invariant
+                        // holds; a stencil outside the grid (an uploaded point at the
+                        // edge) is skipped, as by the other paths.  (One flat test and
+                        // one branch: a nested branch here costs register copies.)
+                        const int rx = s.cx - fx0, ry = s.cy - fy0;
+                        const bool outside = rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2 ||
+                                             !stencil_inside(s.cx, s.cy, P);
+                        if (act && outside) err |= ICE_ERR_POINT_LEFT_AREA;
+                        act = act && !outside;
                         if (act) {
-                            const int rx = s.cx - fx0, ry = s.cy - fy0;
-                            // outside the frame cannot happen while the drift invariant
-                            // holds; a stencil outside the grid (an uploaded point at the
-                            // edge) is skipped, as by the other paths
-                            if (rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2 ||
-                                !stencil_inside(s.cx, s.cy, P)) {
-                                err |= ICE_ERR_POINT_LEFT_AREA;
-                                act = false;
-                            } else {
-                                float wx[3], wy[3];
-                                weights(s.x, wx);
-                                weights(s.y, wy);
-                                float2 nv[3][3];
+                            float wx[3], wy[3];
+                            weights(s.x, wx);
+                            weights(s.y, wy);
+                            float2 nv[3][3];
 #pragma unroll
-                                for (int a = 0; a < 3; a++)
+                            for (int a = 0; a < 3; a++)
 #pragma unroll
-                                    for (int b = 0; b < 3; b++) nv[a][b] = s_v[(rx + a - 1) * Fr::H + (ry + b - 1)];
-                                g2p_gather(nv, s.x, s.y, wx, wy, P.four_h_inv, pvx, pvy, C);
-                                g2p_predict(P, s, pvx, pvy, C, err, vf);
-                            }
+                                for (int b = 0; b < 3; b++) nv[a][b] = s_v[(rx + a - 1) * Fr::H + (ry + b - 1)];
+                            g2p_gather(nv, s.x, s.y, wx, wy, P.four_h_inv, pvx, pvy, C);
+                            g2p_predict(P, s, pvx, pvy, C, err, vf);
                         }
                         bool exact_rm = act && (s.u & kCrushed);
                         if constexpr (SURR) {
                             // the plastic branch through the learned projection, one
                             // wave of points per MFMA batch (all lanes take part)
                             // the plastic branch with the ridge cap active (what it was trained on)
-                            bool want = exact_rm && th == 1.f && s.p_tr >= rw.dpt &&
-                                        s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, rw.dpt, P);
+                            bool want = exact_rm && th == 1.f && s.p_tr >= dpt &&
+                                        s.p_tr >= P.ridge * th * s.Jp && s.q_tr > q_yield_at(s.p_tr, dpt, P);
                             float xr[kMlpIn] = {0.f, 0.f, 1.f, 0.f};
-                            if (want) mlp_features(vf, s.Jp, -rw.dpt * P.tan_phi, xr);
+                            if (want) mlp_features(vf, s.Jp, -dpt * P.tan_phi, xr);
                             if (SA.rec_x) {   // uniform: the recording of trial states
                                 const uint32_t hsh = (uint32_t)mix64(((uint64_t)i << 24) ^
                                                                      (uint64_t)__double_as_longlong(time));
@@ -496,7 +542,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                                 bool accepted = false;
                                 if (want) {
                                     float m0, m1, jpf, pq, qq, viol;
-                                    accepted = mlp_decode(SA.N, y, xr[0], s.Jp, rw.dpt, th, P, SA.tol, m0, m1, jpf, pq, qq, viol);
+                                    accepted = mlp_decode(SA.N, y, xr[0], s.Jp, dpt, th, P, SA.tol, m0, m1, jpf, pq, qq, viol);
                                     if (accepted) {
                                         s.Fe = rebuild(vf, m0, m1);
                                         s.Jp *= jpf;
@@ -513,7 +559,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                             }
                         }
                         if (act) {
-                            if (exact_rm) g2p_return_map(P, s, th, rw.dpt, err, vf);
+                            if (exact_rm) g2p_return_map(P, s, th, dpt, err, vf);
                             if (s.newly_disabled) atomicAdd(&cnt->disabled, 1);
                             bool escaped = false;
                             if (s.cell_updated && (s.u & (kDisabled | kFar)) == kFar) {
@@ -555,6 +601,14 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_pipeline(DevParams P0,
                     }
                 }
                 __syncthreads();
+                if constexpr (PF) {
+                    if (sub + (uint32_t)kStage < n_round) {   // uniform
+                        locate(sub + (uint32_t)kStage, slot_n);
+#pragma unroll
+                        for (int p = 0; p < kPasses; p++)
+                            if (slot_n[p] != ~0u) load_g2p_raw(L, P, slot_n[p], raw_n[p]);
+                    }
+                }
                 // each cell lane sums the staged points sorted into its cell
                 float am[3][3], ax[3][3], ay[3][3];
 #pragma unroll

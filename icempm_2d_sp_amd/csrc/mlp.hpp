@@ -615,6 +615,14 @@ __global__ __launch_bounds__(64) void k_mlp_infer(MlpDev M, MlpNorm N, const flo
 // whether it is accepted at tolerance tol (else the exact map runs).  m0, m1
 // are the new singular values minus one, jpf the factor on Jp_inv.
 //
+// jpf is Je_new / J_tr, as the exact map applies it (kernels.cuh:465), with
+// Je_new = J' cosh D' from the two outputs the violation test checks, not the
+// network's third output (log Jp'/Jp, trained with the others): that output
+// is unchecked, and a relative error of a few percent in it, applied on
+// every plastic substep, moves Jp_inv (and with it the ridge cap) away from
+// the exact run by ~0.5 % over 1000 substeps, while the relation keeps
+// Jp_inv as close as the accepted (p', q').
+//
 // The exact map (kernels.cuh:441-466) picks p1 = min(p_tr, p_ridge) and
 // q1 = q_yield(p1), then builds the new singular values from
 // Je_new = sqrt(1 - 2 p1 / kappa) and t = q1 J_tr / (2 mu) as
@@ -628,7 +636,6 @@ __device__ __forceinline__ bool mlp_decode(const MlpNorm &N, const float y[kMlpO
                                            float &jpf, float &p_out, float &q_out, float &viol) {
     const float lj = -expf(N.ym[0] + N.ys[0] * y[0]);
     const float D = expf(N.ym[1] + N.ys[1] * y[1]);
-    const float dj = N.ym[2] + N.ys[2] * y[2];
     p_out = -P.half_kappa * expm1f(2.f * lj);
     q_out = 2.f * P.mu * sinhf(D);
     // log cosh D' (D'^2 / 2 - D'^4 / 12 below 0.1, where coshf would round to 1)
@@ -640,7 +647,7 @@ __device__ __forceinline__ bool mlp_decode(const MlpNorm &N, const float y[kMlpO
     const float p_ridge = P.ridge * th * jp;
     m0 = expm1f(0.5f * (lj + D));   // new singular values minus one
     m1 = expm1f(0.5f * (lj - D));
-    jpf = expf(dj);
+    jpf = expf((lj + lc) - logJ_tr);   // Je_new / J_tr
     return viol <= tol && p1 <= p_ridge * (1.f + tol) + 1.f && D >= 0.f && __builtin_isfinite(m0) &&
            __builtin_isfinite(m1) && __builtin_isfinite(jpf);
 }

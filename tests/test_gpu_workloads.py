@@ -202,10 +202,12 @@ def test_learned_projection_batch64_compressed_1000_substeps():
            f"{jp_sums[1] / dfe.size:.6f}")
     assert n_acc >= 0.9 * (n_acc + n_fb) and plastic > 0.3
     # Jp_inv integrates the projection's factor over every plastic substep
-    # (~865 of the 1000 per point here), so a per-substep error of a few 1e-6
-    # becomes ~2e-3 per point after 1000; the run as a whole stays on the
-    # exact run's: the median point's state and the mean Jp_inv
-    assert np.median(dfe) <= 1e-4 and np.median(djp) <= 1e-2
-    assert abs(jp_sums[0] / jp_sums[1] - 1) <= 1e-3
+    # (~865 of the 1000 per point here); the factor comes from the projected
+    # state (Je_new / J_tr), so it is as close as the accepted (p', q'): the
+    # median point and the mean Jp_inv stay on the exact run (measured 4e-5
+    # and 2e-5); single points that cross cells at other substeps in the two
+    # runs drift further apart
+    assert np.median(dfe) <= 1e-4 and np.median(djp) <= 1e-3
+    assert abs(jp_sums[0] / jp_sums[1] - 1) <= 1e-4
     t.close()
     ex.close()
