@@ -1005,7 +1005,7 @@ int upload_points_d(IceHandle *h, int d, const float *soa, size_t host_pitch, ui
     }
     D.n = n;
     // the partition's disabled count starts at 0 (gpu_implementation5.cpp:137, gpu_partition.cpp:101-103)
-    const int32_t zero = 0;
+    static const int32_t zero = 0;   // outlives the asynchronous copy
     HIPCHK(h, hipMemcpyAsync(&h->dcnt(d)->disabled, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
     if (n) {
         const float *th = soa + (size_t)ICE_IDX_THICKNESS * host_pitch;
@@ -1261,10 +1261,9 @@ int ice_update_constants(IceHandle *h, const IceParams *p) {
     h->fix = make_fix_scale(h->prm, h->max_thickness);
     frames_stale(h);
     // GPU_Partition::update_constants zeroes the error word (gpu_partition.cpp:143)
-    for (int d = 0; d < h->ndom(); d++) {
-        const uint32_t zero = 0;
+    static const uint32_t zero = 0;   // outlives the asynchronous copies
+    for (int d = 0; d < h->ndom(); d++)
         HIPCHK(h, hipMemcpyAsync(&h->dcnt(d)->err, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     drop_graph(h);
     return ICE_OK;
@@ -2219,10 +2218,9 @@ int ice_mlp_deploy(IceHandle *h, int enable, float tol) {
     if (h->ndom() > 1 && h->mlp_deployed != (enable != 0)) h->tables_dirty = true;   // tile order
     h->mlp_deployed = enable != 0;
     h->mlp_tol = tol;
-    for (int d = 0; d < h->ndom(); d++) {
-        const uint32_t zero2[2] = {0u, 0u};
+    static const uint32_t zero2[2] = {0u, 0u};   // outlives the asynchronous copies
+    for (int d = 0; d < h->ndom(); d++)
         HIPCHK(h, hipMemcpyAsync(&h->dcnt(d)->pad[0], zero2, sizeof(zero2), hipMemcpyHostToDevice, h->stream));
-    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     drop_graph(h);
     return ICE_OK;

@@ -239,8 +239,16 @@ __device__ __forceinline__ void load_g2p_raw(const LT &L, const DevParams &P, ui
 
 // A point's P2G terms into staging slot jj (mass 0 marks an empty slot), and
 // its cell in frame coordinates (column | row << 8; 0xffff if outside).
-__device__ __forceinline__ void stage_terms(float (*s_st)[kStage], uint16_t *s_cw, uint32_t jj, bool stage,
+// Staging position of the batch's k-th point.  The cell lanes read the points
+// of 64 neighbouring cells, about four apart, so that lanes 8 apart would hit
+// one LDS bank; an XOR of bits 5-6 into bits 0-1 (a permutation inside each
+// block of 32) spreads them over distinct banks, and the staging writes of
+// consecutive lanes stay conflict-free.
+__device__ __forceinline__ uint32_t stage_slot(uint32_t k) { return k ^ ((k >> 5) & 3u); }
+
+__device__ __forceinline__ void stage_terms(float (*s_st)[kStage], uint16_t *s_cw, uint32_t jj0, bool stage,
                                             const P2GPoint &q, int fx0, int fy0) {
+    const uint32_t jj = stage_slot(jj0);
     if (stage) {
         const uint32_t lx = (uint32_t)(q.cx - fx0), ly = (uint32_t)(q.cy - fy0);
         s_cw[jj] = (lx < 256u && ly < 256u) ? (uint16_t)(lx | (ly << 8)) : (uint16_t)0xffffu;
@@ -618,7 +626,7 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                 bool any = false;
                 const uint32_t lo = max(f0, sub), hi = min(f0 + flen, sub + (uint32_t)kStage);
                 for (uint32_t jf = lo; jf < hi; jf++) {
-                    const uint32_t k = jf - sub;
+                    const uint32_t k = stage_slot(jf - sub);
                     P2GPoint t;
                     t.mass = s_st[2][k];
                     if (t.mass == 0.f) continue;

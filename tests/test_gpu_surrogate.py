@@ -197,3 +197,52 @@ def test_deployed_surrogate_in_substep(sim):
     compare_points(ps2, pe2, fields=[("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-6)])
     exact.close()
     surr.close()
+
+
+def test_batch_deploy_toggle_matches_exact_batch():
+    """On a batched handle, deploying the projection re-orders the tile table
+    (and refills the pipeline) and withdrawing it orders it back.  Five
+    compressed domains: 16 substeps undeployed, 16 deployed, 16 undeployed
+    again, against the same batch left on the exact map throughout: the
+    first leg matches it at rounding level, the network counts only in the
+    deployed leg (the counts restart at each ice_mlp_deploy), and after the
+    three legs every domain stays within the surrogate's bar of the exact
+    batch (<= 1 % of points beyond 1e-4 in Fe, the rest as for many
+    substeps)."""
+    from icempm_2d_sp_amd.gpu import BatchSimulation
+    from parity import compare_points
+    _print("test_batch_deploy_toggle_matches_exact_batch")
+    doms = scenes.batch64_compressed(seed=707, n_domains=5, min_side=96, max_side=192)
+    t = BatchSimulation(doms)
+    rep = t.train_on_recorded(0, 16, 2_000_000, train_steps=(4000, 3000, 3000))
+    _print("training", {k: rep[k] for k in ("states_seen", "pairs", "pairs_failed", "loss_last")})
+    assert rep["pairs_failed"] == 0
+    t.reset_points()
+    ex = BatchSimulation(doms)
+    many = [("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
+            ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+            ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
+            ("Jp_inv", [abi.IDX_JP_INV], 1e-3)]
+    t.run(0, 16, graph=True)
+    ex.run(0, 16, graph=True)
+    c0 = t.mlp_counts()
+    for d in range(len(doms)):
+        compare_points(t.download(d)[0], ex.download(d)[0], fields=many, max_bad_frac=1e-3, max_flag_mismatch=1e-3)
+    t.mlp_deploy(True, 1e-4)
+    t.run(16, 16, graph=True)
+    c1 = t.mlp_counts()
+    t.mlp_deploy(False)
+    t.run(32, 16, graph=True)
+    c2 = t.mlp_counts()
+    ex.run(16, 32, graph=True)
+    _print("counts before, after the deployed leg, after the last leg", c0, c1, c2)
+    # the counts restart at every ice_mlp_deploy; nothing is projected undeployed
+    assert c0 == (0, 0) and c1[0] > 0 and c2 == (0, 0)
+    for d in range(len(doms)):
+        pt, _, et, dt_ = t.download(d)
+        pe, _, ee, de = ex.download(d)
+        assert et == ee == 0
+        rep = compare_points(pt, pe, fields=many, max_bad_frac=0.01, max_flag_mismatch=0.01)
+        _print("domain", d, "Fe max", rep["Fe"]["max"], "bad", rep["Fe"]["bad"])
+    t.close()
+    ex.close()
