@@ -244,11 +244,14 @@ __device__ __forceinline__ void load_g2p_raw(const LT &L, const DevParams &P, ui
 // one LDS bank; an XOR of bits 5-6 into bits 0-1 (a permutation inside each
 // block of 32) spreads them over distinct banks, and the staging writes of
 // consecutive lanes stay conflict-free.
-__device__ __forceinline__ uint32_t stage_slot(uint32_t k) { return k ^ ((k >> 5) & 3u); }
+// (Not in the surrogate variant, which is at its register budget.)
+template <bool SWZ>
+__device__ __forceinline__ uint32_t stage_slot(uint32_t k) { return SWZ ? k ^ ((k >> 5) & 3u) : k; }
 
+template <bool SWZ>
 __device__ __forceinline__ void stage_terms(float (*s_st)[kStage], uint16_t *s_cw, uint32_t jj0, bool stage,
                                             const P2GPoint &q, int fx0, int fy0) {
-    const uint32_t jj = stage_slot(jj0);
+    const uint32_t jj = stage_slot<SWZ>(jj0);
     if (stage) {
         const uint32_t lx = (uint32_t)(q.cx - fx0), ly = (uint32_t)(q.cy - fy0);
         s_cw[jj] = (lx < 256u && ly < 256u) ? (uint16_t)(lx | (ly << 8)) : (uint16_t)0xffffu;
@@ -419,6 +422,19 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                 f0 = s_pre[cc] + (b - s_rb[cc]);
                 flen = cs[cell + 1] - b;
             }
+            // REG: the round's column starts in registers (rounds of at most 8
+            // columns, i.e. tiles 32 cells high): a point's column is the count
+            // of starts at or below its flattened index, with no chain of
+            // dependent LDS reads (the binary and linear searches over s_pre
+            // cost about five LDS round trips per pass; 0.738 -> 0.709 ms per
+            // launch on the ice field).  Not in the surrogate variant, which is
+            // at its register budget.
+            constexpr bool REG = !SURR && RC <= 8;
+            uint32_t pre_r[REG ? RC : 1];
+            if constexpr (REG) {
+#pragma unroll
+                for (int q = 1; q < RC; q++) pre_r[q] = q < ncol ? s_pre[q] : 0xffffffffu;
+            }
             // this lane's point of each pass of the batch at `base` (slot, or ~0u for none)
             auto locate = [&](uint32_t base, uint32_t (&sl)[kPasses]) {
 #pragma unroll
@@ -426,11 +442,20 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                     sl[p] = ~0u;
                     const uint32_t jw = base + (uint32_t)(p * kTileThreads + wave * 64);
                     if (jw < n_round) {   // wave-uniform
-                        const int cb = flat_col_n(jw, s_pre, ncol);
                         const uint32_t j = jw + lane;
-                        if (j < n_round) {
-                            const int c = col_from_n(j, cb, s_pre, ncol);
-                            sl[p] = s_rb[c] + (j - s_pre[c]);
+                        if constexpr (REG) {
+                            if (j < n_round) {
+                                int c = 0;
+#pragma unroll
+                                for (int q = 1; q < RC; q++) c += pre_r[q] <= j ? 1 : 0;
+                                sl[p] = s_rb[c] + (j - s_pre[c]);
+                            }
+                        } else {
+                            const int cb = flat_col_n(jw, s_pre, ncol);
+                            if (j < n_round) {
+                                const int c = col_from_n(j, cb, s_pre, ncol);
+                                sl[p] = s_rb[c] + (j - s_pre[c]);
+                            }
                         }
                     }
                 }
@@ -588,7 +613,7 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                             const uint64_t bm = __ballot(stage && (s.u & kMoved));
                             if (lane == 0 && bm) atomicAdd(&cnt->n_drift, (uint32_t)__popcll(bm));
                         }
-                        stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
+                        stage_terms<!SURR>(s_st, s_cw, jj, stage, q, fx0, fy0);
                     }
                 } else {
 #pragma unroll
@@ -605,7 +630,7 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                                 else stage = true;
                             }
                         }
-                        stage_terms(s_st, s_cw, jj, stage, q, fx0, fy0);
+                        stage_terms<!SURR>(s_st, s_cw, jj, stage, q, fx0, fy0);
                     }
                 }
                 __syncthreads();
@@ -626,7 +651,7 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                 bool any = false;
                 const uint32_t lo = max(f0, sub), hi = min(f0 + flen, sub + (uint32_t)kStage);
                 for (uint32_t jf = lo; jf < hi; jf++) {
-                    const uint32_t k = stage_slot(jf - sub);
+                    const uint32_t k = stage_slot<!SURR>(jf - sub);
                     P2GPoint t;
                     t.mass = s_st[2][k];
                     if (t.mass == 0.f) continue;
