@@ -648,7 +648,6 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                 for (int a = 0; a < 3; a++)
 #pragma unroll
                     for (int b = 0; b < 3; b++) am[a][b] = ax[a][b] = ay[a][b] = 0.f;
-                bool any = false;
                 const uint32_t lo = max(f0, sub), hi = min(f0 + flen, sub + (uint32_t)kStage);
                 for (uint32_t jf = lo; jf < hi; jf++) {
                     const uint32_t k = stage_slot<!SURR>(jf - sub);
@@ -671,16 +670,6 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                     if ((sx | sy) != 0 && !moved_stencil<TW, TH>(t, sx, sy, cx - fx0, cy - fy0, wx, wy, s_acc, err))
                         continue;
                     p2g_accumulate(t, wx, wy, am, ax, ay);
-                    any = true;
-                }
-                if (any) {
-#pragma unroll
-                    for (int a = 0; a < 3; a++)
-#pragma unroll
-                        for (int b = 0; b < 3; b++) {
-                            if (__builtin_isnan(ax[a][b]) || __builtin_isnan(ay[a][b])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-                            if (__builtin_isnan(am[a][b])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
-                        }
                 }
                 __syncthreads();   // staged terms consumed: the area now takes the cell stencils
                 float *s_cs = &s_st[0][0];   // [27][kTileThreads]: component-major, offset (a, b), cell lane
@@ -723,12 +712,16 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
             }
             __syncthreads();   // the round's ranges are read; the next round may overwrite them
         }
-        // the frame, every node once (coalesced)
+        // the frame, every node once (coalesced).  A NaN P2G contribution
+        // (kernels.cuh:97-98) makes its node's sum NaN, so the sums are checked
+        // here, once per node, rather than per cell stencil and batch.
         float *__restrict__ out = Fn + frame_offset<TW, TH>(dd, tr.tx, tr.ty);
         for (int k = threadIdx.x; k < 3 * Fr::N; k += kTileThreads) {
             const int comp = k / Fr::N, nd = k - comp * Fr::N;
             const int lx = nd / Fr::H, ly = nd - lx * Fr::H;
-            out[k] = s_acc[comp * Fr::NN + lx * Fr::LP + ly];
+            const float v = s_acc[comp * Fr::NN + lx * Fr::LP + ly];
+            if (__builtin_isnan(v)) err |= comp == 0 ? ICE_ERR_GRID_P2G_NAN_MASS : ICE_ERR_GRID_P2G_NAN_VEL;
+            out[k] = v;
         }
         flush_err(cnt, err);
     }

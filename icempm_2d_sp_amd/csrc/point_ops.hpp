@@ -203,9 +203,6 @@ __device__ __forceinline__ void g2p_predict(const DevParams &P, G2PState &s, flo
                                             uint32_t &err, FInv &vf) {
     s.x += pvx * P.dt_h_inv;
     s.y += pvy * P.dt_h_inv;
-    if (__builtin_isnan(pvx) || __builtin_isnan(pvy)) err |= ICE_ERR_POINT_VEL_NAN;
-    if (__builtin_isnan(s.x) || __builtin_isnan(s.y)) err |= ICE_ERR_POINT_POS_NAN;
-    if (isnan4(C)) err |= ICE_ERR_POINT_BP_NAN;
     int dx = 0, dy = 0;
     if (s.x > 0.5f) { s.x -= 1.f; dx = 1; }
     else if (s.x < -0.5f) { s.x += 1.f; dx = -1; }
@@ -230,7 +227,18 @@ __device__ __forceinline__ void g2p_predict(const DevParams &P, G2PState &s, flo
     s.Fe.a01 = e.a01 + (c01 + (c00 * e.a01 + c01 * e.a11));
     s.Fe.a10 = e.a10 + (c10 + (c10 * e.a00 + c11 * e.a10));
     s.Fe.a11 = e.a11 + (c11 + (c10 * e.a01 + c11 * e.a11));
-    if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
+    // the NaN flags of velocity, position, Bp and Fe (kernels.cuh:242-244,
+    // 273): one test of their sum on the common path (a NaN anywhere makes it
+    // NaN, and the cell shift above leaves a NaN position NaN; an inf - inf
+    // only sends the point to the exact tests), the flags themselves only
+    // where it fails
+    if (__builtin_isnan(((pvx + pvy) + (s.x + s.y)) + ((C.a00 + C.a01) + (C.a10 + C.a11)) +
+                        ((s.Fe.a00 + s.Fe.a01) + (s.Fe.a10 + s.Fe.a11)))) {
+        if (__builtin_isnan(pvx) || __builtin_isnan(pvy)) err |= ICE_ERR_POINT_VEL_NAN;
+        if (__builtin_isnan(s.x) || __builtin_isnan(s.y)) err |= ICE_ERR_POINT_POS_NAN;
+        if (isnan4(C)) err |= ICE_ERR_POINT_BP_NAN;
+        if (isnan4(s.Fe)) err |= ICE_ERR_POINT_FE_NAN;
+    }
     vf = invariants(s.Fe);
     compute_pq(vf, P, s.p_tr, s.q_tr);
     if (!(s.u & kCrushed) && outside_failure_surface(s.p_tr, s.q_tr, P)) s.u |= kCrushed;
