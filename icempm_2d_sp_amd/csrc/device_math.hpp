@@ -47,7 +47,7 @@ constexpr uint32_t kInternalBits = kMoved | kEscaped | kFresh | kDriftBits | kFa
 __device__ __forceinline__ void drift_cell(uint32_t &u, int dx, int dy) {
     const int nx = (((int)(u << 8)) >> 30) + dx;   // bits 22-23, sign-extended
     const int ny = (((int)(u << 6)) >> 30) + dy;   // bits 24-25
-    if (nx < -1 || nx > 1 || ny < -1 || ny > 1) {
+    if ((uint32_t)(nx + 1) > 2u || (uint32_t)(ny + 1) > 2u) {   // outside -1..1
         u |= kFar;
         return;
     }

@@ -523,8 +523,8 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                         // edge) is skipped, as by the other paths.  (One flat test and
                         // one branch: a nested branch here costs register copies.)
                         const int rx = s.cx - fx0, ry = s.cy - fy0;
-                        const bool outside = rx < 1 || rx > Fr::W - 2 || ry < 1 || ry > Fr::H - 2 ||
-                                             !stencil_inside(s.cx, s.cy, P);
+                        const bool outside = (uint32_t)(rx - 1) > (uint32_t)(Fr::W - 3) ||
+                                             (uint32_t)(ry - 1) > (uint32_t)(Fr::H - 3) || !stencil_inside(s.cx, s.cy, P);
                         if (act && outside) err |= ICE_ERR_POINT_LEFT_AREA;
                         act = act && !outside;
                         if (act) {

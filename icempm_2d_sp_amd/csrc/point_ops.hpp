@@ -214,12 +214,11 @@ __device__ __forceinline__ void g2p_predict(const DevParams &P, G2PState &s, flo
         s.cy += dy;
         drift_cell(s.u, dx, dy);   // the sorted path's bookkeeping; no effect on the state
     }
-    s.newly_disabled = false;
-    if (s.cell_updated && (s.cx <= 1 || s.cx >= P.gx - 2 || s.cy <= 1 || s.cy >= P.gy - 2)) {
-        s.u |= kDisabled;
-        s.newly_disabled = true;
-    }
-    if (s.x > 0.5f || s.x < -0.5f || s.y > 0.5f || s.y < -0.5f) err |= ICE_ERR_POINT_JUMP_CELLS;
+    // the disable band, c <= 1 or c >= g - 2, as one unsigned compare per axis
+    s.newly_disabled = s.cell_updated && ((uint32_t)(s.cx - 2) > (uint32_t)(P.gx - 5) ||
+                                          (uint32_t)(s.cy - 2) > (uint32_t)(P.gy - 5));
+    if (s.newly_disabled) s.u |= kDisabled;
+    if (fmaxf(fabsf(s.x), fabsf(s.y)) > 0.5f) err |= ICE_ERR_POINT_JUMP_CELLS;
     // Fe = (I + dt C) Fe on e = Fe - I: e' = e + dt C + dt C e
     const M2 e = s.Fe;
     const float c00 = P.dt * C.a00, c01 = P.dt * C.a01, c10 = P.dt * C.a10, c11 = P.dt * C.a11;
@@ -336,7 +335,8 @@ __device__ __forceinline__ void load_g2p_state(const Lines &L, uint32_t i, uint3
 // but still scatters; here such points are flagged and skipped, so no access
 // leaves the grid.
 __device__ __forceinline__ bool stencil_inside(int cx, int cy, const DevParams &P) {
-    return cx >= 1 && cy >= 1 && cx <= P.gx - 2 && cy <= P.gy - 2;
+    // 1 <= c <= g - 2 as one unsigned compare per axis (g >= 5 for every grid)
+    return (uint32_t)(cx - 1) <= (uint32_t)(P.gx - 3) && (uint32_t)(cy - 1) <= (uint32_t)(P.gy - 3);
 }
 
 // OR an error word into the device counter once per wave.
