@@ -122,7 +122,9 @@ int ice_allocate(IceHandle *h, uint32_t n_points_capacity);
  * lines (19, or 20 with the cohesion line).  Sets the disabled count to 0, as
  * split_hssoa_into_partitions does (gpu_implementation5.cpp:137).  Unless
  * ICE_FLAG_NAIVE is set, the points are then put in cell order on the device;
- * ICE_IDX_POINT identifies them in later downloads. */
+ * ICE_IDX_POINT identifies them in later downloads.  The device keeps Fe - I
+ * in the four Fe lines (small strains keep their FP32 precision); the
+ * downloads add I back, so callers read and write Fe. */
 int ice_upload_points(IceHandle *h, const float *soa, size_t host_pitch, uint32_t n, int n_lines);
 /* GPU_Partition::transfer_grid_data_to_device (gpu_partition.cpp:108-137):
  * status [gx*gy] bytes, node index x*gy+y; normals [2][gx*gy] or NULL. */
