@@ -125,6 +125,7 @@ SIGNATURES = {
     "ice_mlp_default_sampling": (C.c_int, [C.c_void_p]),
     "ice_mlp_generate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(C.c_uint32)]),
+    "ice_upload_wind_frames": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double]),
     "ice_mlp_record": (C.c_int, [_P, C.c_uint32, C.c_float]),
     "ice_mlp_record_keep": (C.c_int, [_P, C.c_float]),
     "ice_mlp_record_count": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),

@@ -35,8 +35,10 @@ namespace {
 
 std::string g_create_error;
 
-constexpr int kGridLines = 11;   // m px py nx ny cur_x cur_y fx fy wind_x wind_y
-enum { G_M = 0, G_PX, G_PY, G_NX, G_NY, G_CX, G_CY, G_FX, G_FY, G_WX, G_WY };
+// m px py nx ny cur_x cur_y fx fy wind_x wind_y, and the time-varying wind's
+// frames: w0 and w1 - w0 (ice_upload_wind_frames)
+constexpr int kGridLines = 15;
+enum { G_M = 0, G_PX, G_PY, G_NX, G_NY, G_CX, G_CY, G_FX, G_FY, G_WX, G_WY, G_W0X, G_W0Y, G_DWX, G_DWY };
 constexpr int kChunk = 16;       // substeps per captured graph
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -55,6 +57,10 @@ struct Domain {
     bool has_current = false;
     bool sorted = false;
     int steps_since_sort = 0, n_sorts = 0;
+    // time-varying wind (ice_upload_wind_frames): wind = w0 + s (w1 - w0),
+    // s = clamp((t - wind_t0) * wind_inv_span, 0, 1)
+    bool wind_frames = false;
+    double wind_t0 = 0.0, wind_inv_span = 0.0;
 };
 
 }  // namespace
@@ -804,9 +810,26 @@ int phase_p2g_pipelined(IceHandle *h) {
     return ICE_OK;
 }
 
+// The wind-stress lines at the substep's time for every domain with
+// time-varying wind, before its node update reads them.
+int wind_frames_apply(IceHandle *h, const StepArg &S) {
+    for (int d = 0; d < h->ndom(); d++) {
+        const Domain &D = h->dom[d];
+        if (!D.wind_frames || D.N == 0) continue;
+        hipLaunchKernelGGL(k_wind_frames, dim3((unsigned)std::min<size_t>(blocks_for(D.N, 256), 4096)), dim3(256), 0,
+                           h->stream, S, h->dp, h->dgline(G_WX, d), h->dgline(G_WY, d),
+                           (const float *)h->dgline(G_W0X, d), (const float *)h->dgline(G_W0Y, d),
+                           (const float *)h->dgline(G_DWX, d), (const float *)h->dgline(G_DWY, d), D.N, D.wind_t0,
+                           D.wind_inv_span);
+        LAUNCHCHK(h, "k_wind_frames");
+    }
+    return ICE_OK;
+}
+
 int substep_args(IceHandle *h, const StepArg &S) {
     rec(h, 0);
     int rc;
+    if ((rc = wind_frames_apply(h, S))) return rc;
     if (h->pipelined()) {
         if ((rc = phase_pipelined(h, S))) return rc;
     } else {
@@ -1051,6 +1074,7 @@ int upload_forcing_d(IceHandle *h, int d, const float *current, const float *win
         if ((rc = launch_fill(h, h->dgline(G_CY, d), D.N))) return rc;
         D.has_current = false;
     }
+    D.wind_frames = false;   // a static wind field replaces any time-varying one
     if (wind) {
         HIPCHK(h, hipMemcpyAsync(h->dgline(G_WX, d), wind, D.N * sizeof(float), hipMemcpyHostToDevice, h->stream));
         HIPCHK(h, hipMemcpyAsync(h->dgline(G_WY, d), wind + D.N, D.N * sizeof(float), hipMemcpyHostToDevice, h->stream));
@@ -1237,6 +1261,28 @@ int ice_upload_forcing(IceHandle *h, const float *current, const float *wind) {
     return upload_forcing_d(h, 0, current, wind);
 }
 
+int ice_upload_wind_frames(IceHandle *h, int d, const float *wind0, const float *wind1, double t0, double t1) {
+    if (!h || !wind0 || !wind1 || !(t1 > t0) || !std::isfinite(t0) || !std::isfinite(t1)) return ICE_E_ARG;
+    if (check_domain(h, d)) return ICE_E_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    Domain &D = h->dom[d];
+    const size_t N = D.N;
+    std::vector<float> dw(2 * N);
+    for (size_t i = 0; i < 2 * N; i++) dw[i] = wind1[i] - wind0[i];
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // no substep in flight reads the lines
+    for (int c = 0; c < 2; c++) {
+        HIPCHK(h, hipMemcpy(h->dgline(c ? G_W0Y : G_W0X, d), wind0 + c * N, N * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dgline(c ? G_WY : G_WX, d), wind0 + c * N, N * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dgline(c ? G_DWY : G_DWX, d), dw.data() + c * N, N * sizeof(float),
+                            hipMemcpyHostToDevice));
+    }
+    D.wind_frames = true;
+    D.wind_t0 = t0;
+    D.wind_inv_span = 1.0 / (t1 - t0);
+    drop_graph(h);   // captured substeps must now include the wind update
+    return ICE_OK;
+}
+
 int ice_upload_domain(IceHandle *h, int d, const float *soa, size_t host_pitch, uint32_t n, int n_lines,
                       const uint8_t *status, const float *current, const float *wind_stress) {
     if (!h) return ICE_E_ARG;
@@ -1339,6 +1385,8 @@ int ice_update_nodes(IceHandle *h, float simulation_time, float windSpeed, float
     if (h->pipelined()) return ICE_OK;   // fused into the tile pass of ice_g2p
     StepArg S{(double)simulation_time, 0, nullptr, 0};
     frames_stale(h);
+    int rc = wind_frames_apply(h, S);
+    if (rc) return rc;
     return phase_update(h, S);
 }
 
@@ -1351,7 +1399,8 @@ int ice_g2p(IceHandle *h, int recordPQ, int enablePointTransfer, int applyGlensL
     h->last_t = h->pending_t;
     int rc;
     if (h->pipelined()) {
-        rc = phase_pipelined(h, S);
+        rc = wind_frames_apply(h, S);
+        if (!rc) rc = phase_pipelined(h, S);
     } else {
         frames_stale(h);
         rc = phase_g2p(h, S);

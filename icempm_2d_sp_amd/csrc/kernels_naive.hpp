@@ -66,6 +66,26 @@ __global__ void k_step_set(Counters *cnt, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) cnt->step = v;
 }
 
+// The wind-stress lines of a domain at the substep's time from its two
+// frames: w0 + s (w1 - w0), s = clamp((t - t0) / (t1 - t0), 0, 1), the time
+// interpolation the reference sketches and leaves unfinished (kernels.cuh:580-622,
+// (1 - tb) w0 + tb w1; windandcurrentinterpolator.cpp:66-70).
+__global__ __launch_bounds__(256) void k_wind_frames(StepArg S, DevParams P, float *__restrict__ wx,
+                                                     float *__restrict__ wy, const float *__restrict__ w0x,
+                                                     const float *__restrict__ w0y, const float *__restrict__ dwx,
+                                                     const float *__restrict__ dwy, size_t n, double t0,
+                                                     double inv_span) {
+    double t;
+    bool rec;
+    resolve_step(S, P, t, rec);
+    const float s = (float)fmin(fmax((t - t0) * inv_span, 0.0), 1.0);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        wx[i] = w0x[i] + s * dwx[i];
+        wy[i] = w0y[i] + s * dwy[i];
+    }
+}
+
 // Fe -> Fe - I on the first n points of an uploaded domain (the device keeps
 // e = Fe - I in the Fe lines; download adds I back on the host).
 __global__ __launch_bounds__(256) void k_fe_minus_identity(float *__restrict__ pts, size_t pitch, uint32_t n) {

@@ -293,7 +293,18 @@ class GPU_Implementation5:
             p.record_timings(enablePointTransfer)
 
 
-class SurrogateMixin:
+class ForcingMixin:
+    def upload_wind_frames(self, wind0, wind1, t0: float, t1: float, d: int = 0):
+        """Time-varying wind stress of domain d: frames [2][gx*gy] (Pa) valid at
+        t0 < t1, interpolated linearly in time by every later substep
+        (ice_upload_wind_frames)."""
+        a = np.ascontiguousarray(wind0, np.float32)
+        b = np.ascontiguousarray(wind1, np.float32)
+        self._chk(self.lib.ice_upload_wind_frames(self.h, d, abi.ptr(a), abi.ptr(b), float(t0), float(t1)),
+                  "ice_upload_wind_frames")
+
+
+class SurrogateMixin(ForcingMixin):
     """The learned plastic projection's entry points (ice_mlp_*) on a handle."""
 
     def mlp_generate(self, n: int, seed: int = 1, sampling: "abi.IceMlpSampling | None" = None, copy: bool = False):

@@ -133,6 +133,15 @@ int ice_upload_grid(IceHandle *h, const uint8_t *status, const float *normals);
  * wind-stress field the reference declares but never applies
  * (gpu_implementation5.cpp:51-71): each [2][gx*gy] (x then y) or NULL = zero. */
 int ice_upload_forcing(IceHandle *h, const float *current, const float *wind_stress);
+/* Time-varying wind, the forcing the reference leaves unfinished
+ * (WindAndCurrentInterpolator::SetTime, windandcurrentinterpolator.cpp:66-70;
+ * the commented-out interpolation of kernels.cuh:580-622, (1 - tb) w0 + tb w1):
+ * two wind-stress frames [2][gx*gy] (x then y, Pa) of domain d, valid at
+ * t0 < t1.  Every later substep's node update uses w0 + s (w1 - w0) with
+ * s = clamp((t - t0) / (t1 - t0), 0, 1) at its simulation time t.  A caller
+ * stepping through a series uploads the next pair when t passes t1;
+ * ice_upload_forcing (a static wind field) ends it. */
+int ice_upload_wind_frames(IceHandle *h, int domain, const float *wind0, const float *wind1, double t0, double t1);
 /* GPU_Partition::update_constants (gpu_partition.cpp:140-170): new constants,
  * same grid size. */
 int ice_update_constants(IceHandle *h, const IceParams *p);

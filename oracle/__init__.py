@@ -44,6 +44,7 @@ def load():
         "orc_destroy": (None, [P]),
         "orc_set_points": (C.c_int, [P, P, C.c_size_t, C.c_uint32, C.c_int]),
         "orc_set_grid": (C.c_int, [P, P, P, P]),
+        "orc_set_wind_frames": (C.c_int, [P, P, P, C.c_double, C.c_double]),
         "orc_substep": (C.c_int, [P, C.c_double, C.c_int]),
         "orc_run": (C.c_int, [P, C.c_int, C.c_int]),
         "orc_bench": (C.c_double, [P, C.c_int, C.c_int]),
@@ -118,6 +119,14 @@ class Oracle:
         cur = None if current is None else np.ascontiguousarray(current, np.float32)
         wnd = None if wind is None else np.ascontiguousarray(wind, np.float32)
         self.lib.orc_set_grid(self.h, _p(st), _p(cur), _p(wnd))
+
+    def set_wind_frames(self, w0, w1, t0: float, t1: float):
+        """Time-varying wind: frames [2][gx*gy] valid at t0 < t1 (static wind
+        from set_grid ends it)."""
+        a = np.ascontiguousarray(w0, np.float32)
+        b = np.ascontiguousarray(w1, np.float32)
+        if self.lib.orc_set_wind_frames(self.h, _p(a), _p(b), float(t0), float(t1)):
+            raise ValueError("wind frames need t1 > t0")
 
     @classmethod
     def from_scene(cls, scene, precision: int = 64):

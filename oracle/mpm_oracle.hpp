@@ -295,6 +295,10 @@ template <class T> struct State {
     std::vector<T> P, Q, Jp, px, py, vx, vy, F00, F01, F10, F11, B00, B01, B10, B11, thick, coh;
     // grid
     std::vector<T> gm, gpx, gpy, gvx, gvy, gfx, gfy, cur_x, cur_y, wind_x, wind_y;
+    // time-varying wind: wind_x/y hold the frame w0, wind_dx/dy = w1 - w0
+    std::vector<T> wind_dx, wind_dy;
+    bool wind_frames = false;
+    double wind_t0 = 0, wind_t1 = 1;
     std::vector<uint8_t> status;
     uint32_t error_code = 0;
     int disabled = 0;
@@ -308,7 +312,8 @@ template <class T> struct State {
         gx = p.GridXTotal; gy = p.GridYTotal;
         has_cohesion = (p.flags & ICE_FLAG_COHESION_LINE) != 0;
         size_t N = (size_t)gx * gy;
-        for (auto *v : {&gm, &gpx, &gpy, &gvx, &gvy, &gfx, &gfy, &cur_x, &cur_y, &wind_x, &wind_y}) v->assign(N, T(0));
+        for (auto *v : {&gm, &gpx, &gpy, &gvx, &gvy, &gfx, &gfy, &cur_x, &cur_y, &wind_x, &wind_y, &wind_dx, &wind_dy})
+            v->assign(N, T(0));
         status.assign(N, (uint8_t)ICE_GRID_STATUS_WATER);
     }
     void resize(size_t np) {
@@ -401,7 +406,11 @@ template <class T> struct State {
 
     // partition_kernel_update_nodes (kernels.cuh:109-177) plus the wind-stress
     // term (not in the reference): v += dt * windStressToAccel * tau_wind on water nodes,
-    // after the water drag and before the velocity clamp.
+    // after the water drag and before the velocity clamp.  With time-varying
+    // wind, tau_wind = w0 + s (w1 - w0), s = clamp((t - t0) / (t1 - t0), 0, 1):
+    // the time interpolation (1 - tb) w0 + tb w1 of the reference's unfinished
+    // wind interpolation (kernels.cuh:580-622, commented out;
+    // windandcurrentinterpolator.cpp:66-70).
     void update_nodes(double simulation_time) {
         const T dt = T(prm.InitialTimeStep);
         const T vmax = T(d.vmax);
@@ -409,6 +418,8 @@ template <class T> struct State {
         const T kQp = T(prm.waterDragEffectiveQuadratic * prm.InitialTimeStep);
         const T wacc = T(prm.windStressToAccel);
         const T t = T(simulation_time);
+        const T ws = wind_frames ? T(std::min(std::max((simulation_time - wind_t0) / (wind_t1 - wind_t0), 0.0), 1.0))
+                                 : T(0);
         T ramp;
         if (t < T(300000)) ramp = T(1) + std::min(t / T(3600 * 2), T(2));
         else ramp = T(3) + std::min((t - T(300000)) / T(3600 * 10), T(2));
@@ -432,8 +443,10 @@ template <class T> struct State {
                 k = std::min(k, T(0.1));
                 vx_ += k * ux;
                 vy_ += k * uy;
-                vx_ += dt * wacc * wind_x[idx];
-                vy_ += dt * wacc * wind_y[idx];
+                const T tx = wind_frames ? wind_x[idx] + ws * wind_dx[idx] : wind_x[idx];
+                const T ty = wind_frames ? wind_y[idx] + ws * wind_dy[idx] : wind_y[idx];
+                vx_ += dt * wacc * tx;
+                vy_ += dt * wacc * ty;
             }
             if (vx_ * vx_ + vy_ * vy_ > vmax * vmax * T(0.1)) { vx_ = 0; vy_ = 0; }
             gvx[idx] = vx_;

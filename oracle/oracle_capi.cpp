@@ -1,5 +1,6 @@
 // oracle_capi.cpp — extern "C" face of the oracle for the Python tests and the
 // CPU baseline.  TEST INFRASTRUCTURE ONLY: never linked into the product.
+#include <type_traits>
 #include "mpm_oracle.hpp"
 #include "mlp_oracle.hpp"
 #include "frame_oracle.hpp"
@@ -121,6 +122,25 @@ int orc_set_grid(void *h, const uint8_t *status, const float *current, const flo
             s.cur_x[i] = current ? current[i] : 0; s.cur_y[i] = current ? current[N + i] : 0;
             s.wind_x[i] = wind ? wind[i] : 0; s.wind_y[i] = wind ? wind[N + i] : 0;
         }
+        s.wind_frames = false;
+    });
+    return 0;
+}
+
+// time-varying wind: frames w0, w1 [2][gx*gy] floats valid at t0 < t1
+int orc_set_wind_frames(void *h, const float *w0, const float *w1, double t0, double t1) {
+    if (!(t1 > t0)) return 1;
+    DISPATCH(h, {
+        size_t N = (size_t)s.gx * s.gy;
+        using VT = typename std::decay<decltype(s.wind_dx)>::type::value_type;   // the state's real type
+        for (size_t i = 0; i < N; i++) {
+            s.wind_x[i] = w0[i]; s.wind_y[i] = w0[N + i];
+            s.wind_dx[i] = VT(w1[i]) - s.wind_x[i];
+            s.wind_dy[i] = VT(w1[N + i]) - s.wind_y[i];
+        }
+        s.wind_frames = true;
+        s.wind_t0 = t0;
+        s.wind_t1 = t1;
     });
     return 0;
 }

@@ -485,6 +485,43 @@ def test_update_nodes_closed_forms():
     assert np.count_nonzero(frc) == 2
 
 
+def test_update_time_varying_wind_closed_form():
+    """Time-varying wind: between frames w0 (at t0) and w1 (at t1) the node's
+    wind acceleration is dt * windStressToAccel * ((1 - s) w0 + s w1),
+    s = clamp((t - t0) / (t1 - t0), 0, 1) -- the time interpolation of the
+    reference's unfinished wind interpolator (kernels.cuh:580-622, commented
+    out).  Drag off, one water node at rest; before t0 the frame w0, after t1
+    the frame w1; static wind from set_grid ends it."""
+    prm = sea_ice_params(8, 8).replace(waterDragEffectiveQuadratic=0.0, waterDragEffectiveLinear=0.0)
+    prm.ParticleVolume = 0.25
+    o = oracle.Oracle(prm, 64)
+    N = 64
+    o.set_points(np.zeros((19, 0), np.float32))
+    o.set_grid(np.full(N, 100, np.uint8))
+    w0 = np.zeros((2, N), np.float32)
+    w1 = np.zeros((2, N), np.float32)
+    w0[0, 3], w0[1, 3] = 0.2, -0.1
+    w1[0, 3], w1[1, 3] = -0.4, 0.3
+    t0, t1 = 100.0, 300.0
+    o.set_wind_frames(w0, w1, t0, t1)
+    acc = prm.InitialTimeStep * prm.windStressToAccel
+    for t, s in [(50.0, 0.0), (100.0, 0.0), (150.0, 0.25), (200.0, 0.5), (300.0, 1.0), (900.0, 1.0)]:
+        m = np.zeros(N); m[3] = 1.0
+        o.set_grid_state(m=m)
+        o.update_nodes(t)
+        _, vel, _ = o.grid()
+        want = acc * ((1 - s) * w0[:, 3].astype(np.float64) + s * w1[:, 3].astype(np.float64))
+        np.testing.assert_allclose(vel[:, 3], want, rtol=1e-14, atol=0)
+    with pytest.raises(ValueError):
+        o.set_wind_frames(w0, w1, t1, t0)
+    o.set_grid(np.full(N, 100, np.uint8), None, w0)   # static wind again
+    m = np.zeros(N); m[3] = 1.0
+    o.set_grid_state(m=m)
+    o.update_nodes(200.0)
+    _, vel, _ = o.grid()
+    np.testing.assert_allclose(vel[:, 3], acc * w0[:, 3].astype(np.float64), rtol=1e-14)
+
+
 def test_update_drag_cap():
     prm = sea_ice_params(8, 8).replace(waterDragEffectiveQuadratic=1e4)
     o = oracle.Oracle(prm, 64)
