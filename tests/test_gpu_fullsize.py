@@ -1,5 +1,5 @@
 """The plain substep at full size (icefield 4096x2048, 31.8M points, the
-benchmark's workload): against the FP64 oracle over 16 substeps, and
+benchmark's workload): against the FP64 oracle over 16 and 500 substeps, and
 through properties that hold at any size:
 P2G conserves mass and momentum (the weights have unit sum and zero first
 moment, so sum_i m_i = sum_p m_p and sum_i p_i = sum_p m_p v_p), no error
@@ -77,30 +77,60 @@ def test_sort_at_full_size(sim):
     assert perm.shape[0] == n and np.unique(perm).shape[0] == n
 
 
+def _mean_live_vx(pts):
+    live = (pts[abi.IDX_UTILITY].view(np.uint32) & abi.STATUS_DISABLED) == 0
+    return float(pts[abi.IDX_VELX][live].astype(np.float64).mean())
+
+
+@pytest.mark.timeout(600)
 def test_icefield_matches_fp64_oracle_at_full_size():
     """The benchmark's workload itself (31.8M points, lognormal cohesion, GRF
-    wind) for 16 graph-replayed substeps against the FP64 oracle (OpenMP on
-    the host) run from the same scene: position, velocity and Fe within 1e-4
-    of the field scale and Jp_inv within 1e-3 on all but 1e-5 of the points,
-    utility and cell words equal on all but 1e-5, identical error word and
-    disabled count.  (Measured: velocity within 1.2e-6 of its scale, Fe at
-    FP32 rounding, one point of 31.8M on the other side of a cell edge.)"""
+    wind) for 500 graph-replayed substeps (its configured run length, 0.05 s)
+    against the FP64 oracle (OpenMP on the host) run from the same scene.
+    After 16 substeps: position, velocity and Fe within 1e-4 of the field
+    scale and Jp_inv within 1e-3 on all but 1e-5 of the points, utility and
+    cell words equal on all but 1e-5.  After 500: the same tolerances on all
+    but 1e-4 of the points.  Identical error word and disabled count at both.
+    (Measured after 16: velocity within 1.2e-6 of its scale, Fe at FP32
+    rounding, one point of 31.8M on the other side of a cell edge; after 500:
+    velocity within 3.0e-6, Fe - I within 2.3e-13, 9 points on the other side
+    of a cell edge, mean x velocity 1.06490e-5 m/s on both.)  The oracle
+    takes ~0.2 s per substep on 16 cores, ~100 s in all."""
     import oracle
     from parity import compare_points
     _print("test_icefield_matches_fp64_oracle_at_full_size")
     sc = scenes.icefield_4096x2048()
     s = Simulation(sc)
     s.run(0, 16, graph=True)
-    pts, _, err, dis = s.download()
+    pts16, _, err16, dis16 = s.download()
+    s.run(16, 484, graph=True)
+    pts500, _, err500, dis500 = s.download()
     s.close()
+    fields = [("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
+              ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+              ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
+              ("Jp_inv", [abi.IDX_JP_INV], 1e-3)]
+
+    def show(k, rep):
+        _print(k, {n: (v if not isinstance(v, dict) else {kk: f"{vv:.3e}" for kk, vv in v.items()})
+                   for n, v in rep.items()})
+
     o = oracle.Oracle.from_scene(sc, 64)
     o.run(0, 16)
+    rep = {}
+    try:
+        compare_points(pts16, o.points(), fields=fields, max_bad_frac=1e-5, max_flag_mismatch=1e-5, report=rep)
+    finally:
+        show("after 16 substeps", rep)
+    assert err16 == o.error == 0 and dis16 == o.disabled
+    del pts16
+    o.run(16, 484)
     ref = o.points()
-    rep = compare_points(pts, ref, fields=[
-        ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
-        ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
-        ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
-        ("Jp_inv", [abi.IDX_JP_INV], 1e-3)], max_bad_frac=1e-5, max_flag_mismatch=1e-5)
-    _print({k: (v if not isinstance(v, dict) else {kk: f"{vv:.3e}" for kk, vv in v.items()}) for k, v in rep.items()})
-    assert err == o.error == 0 and dis == o.disabled
+    _print("after 500 substeps: mean x velocity device", _mean_live_vx(pts500), "oracle", _mean_live_vx(ref))
+    rep = {}
+    try:
+        compare_points(pts500, ref, fields=fields, max_bad_frac=1e-4, max_flag_mismatch=1e-4, report=rep)
+    finally:
+        show("after 500 substeps", rep)
+    assert err500 == o.error == 0 and dis500 == o.disabled
     o.close()
