@@ -162,7 +162,10 @@ def main() -> int:
         out = {"bytes": nbytes}
         for mode, name, moved in [(0, "copy", 2 * nbytes), (1, "read", nbytes), (2, "write", nbytes)]:
             ms = C.c_double(0)
-            abi.check(abi.load().ice_stream_bench(0, nbytes, 20, mode, C.byref(ms)), None, "ice_stream_bench")
+            cal = abi.load_calib()
+            rc = cal.ice_stream_bench(0, nbytes, 20, mode, C.byref(ms))
+            if rc != 0:
+                raise abi.IceError(f"ice_stream_bench failed ({rc}): {cal.ice_calib_last_error().decode()}")
             gbs = moved / (ms.value * 1e-3) / 1e9
             out[name] = {"ms": ms.value, "GBs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
         print(json.dumps(out), flush=True)

@@ -120,8 +120,6 @@ SIGNATURES = {
     "ice_set_tile": (C.c_int, [_P, C.c_int, C.c_int]),
     "ice_set_option": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "ice_get_info": (C.c_int, [_P, C.POINTER(IceInfo)]),
-    "ice_copy_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
-    "ice_stream_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "ice_mlp_default_sampling": (C.c_int, [C.c_void_p]),
     "ice_mlp_generate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(C.c_uint32)]),
@@ -169,6 +167,35 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         f.restype = res
         f.argtypes = args
     _lib = lib
+    return lib
+
+
+CALIB_PATH = os.path.join(HERE, "lib", "libicempm_calib.so")
+
+# every symbol include/icempm_calib.h declares
+CALIB_SIGNATURES = {
+    "ice_stream_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "ice_copy_bench": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "ice_calib_last_error": (C.c_char_p, []),
+}
+
+_calib = None
+
+
+def load_calib(path: str = CALIB_PATH) -> C.CDLL:
+    """Load libicempm_calib.so (the HBM streaming measurements) and bind its
+    entry points; raises if it is missing."""
+    global _calib
+    if _calib is not None:
+        return _calib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(path)
+    for name, (res, args) in CALIB_SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _calib = lib
     return lib
 
 

@@ -1,4 +1,6 @@
-"""Build the gfx950 HIP library in-tree: icempm_2d_sp_amd/lib/libicempm.so.
+"""Build the gfx950 HIP library in-tree: icempm_2d_sp_amd/lib/libicempm.so
+(with libicempm_calib.so, the C++ model driver and, where HDF5 is found,
+libicempm_io.so).
 
 hipcc compiles for gfx950 only; packed FP32 instructions are disabled for all
 device code.  `python -m icempm_2d_sp_amd.build` builds from the command line.
@@ -57,9 +59,27 @@ def build_library(verbose: bool = False) -> str:
     if verbose and res.stderr.strip():
         print(res.stderr, file=sys.stderr)
     os.replace(tmp, LIB)
+    build_calib(verbose)
     build_driver(verbose)
     build_io(verbose)
     return LIB
+
+
+CALIB_LIB = os.path.join(LIBDIR, "libicempm_calib.so")
+
+
+def build_calib(verbose: bool = False) -> str:
+    """libicempm_calib.so: the HBM streaming measurements (copy, read, write
+    kernels of known traffic) that calibrate bandwidth and profiler counters."""
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "calib.hip"),
+           "-o", CALIB_LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"libicempm_calib.so build failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+    os.replace(CALIB_LIB + ".tmp", CALIB_LIB)
+    return CALIB_LIB
 
 
 IO_LIB = os.path.join(LIBDIR, "libicempm_io.so")

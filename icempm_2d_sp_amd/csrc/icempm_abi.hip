@@ -1744,50 +1744,6 @@ int ice_autotune(IceHandle *h, int nsteps, int *tile_w, int *tile_h, double *bes
     return upload_tables(h);
 }
 
-int ice_stream_bench(int device, size_t bytes, int reps, int mode, double *ms_per_pass) {
-    if (!ms_per_pass || reps <= 0 || bytes < 1024 || mode < 0 || mode > 2) return ICE_E_ARG;
-    *ms_per_pass = 0.0;
-    if (hipSetDevice(device) != hipSuccess) { g_create_error = "hipSetDevice failed"; return ICE_E_DEVICE; }
-    const size_t n4 = bytes / 16;
-    const unsigned blocks = 8192, threads = 256;
-    float4 *a = nullptr, *b = nullptr;
-    hipStream_t st = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = ICE_OK;
-    auto ok = [&](hipError_t e) {
-        if (e != hipSuccess && rc == ICE_OK) { g_create_error = hipGetErrorString(e); rc = ICE_E_HIP; }
-        return rc == ICE_OK;
-    };
-    // b: the copy destination, or the read kernel's sink (one float per thread)
-    const size_t b_bytes = mode == 1 ? (size_t)blocks * threads * sizeof(float) : n4 * 16;
-    auto launch = [&]() {
-        if (mode == 0) hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(threads), 0, st, a, b, n4);
-        else if (mode == 1) hipLaunchKernelGGL(k_read4, dim3(blocks), dim3(threads), 0, st, a, n4, (float *)b);
-        else hipLaunchKernelGGL(k_write4, dim3(blocks), dim3(threads), 0, st, a, n4);
-    };
-    if (ok(hipMalloc(&a, n4 * 16)) && ok(hipMalloc(&b, b_bytes)) && ok(hipStreamCreate(&st)) &&
-        ok(hipEventCreate(&e0)) && ok(hipEventCreate(&e1)) && ok(hipMemsetAsync(a, 0, n4 * 16, st))) {
-        launch();   // warm-up
-        ok(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; r++) launch();
-        ok(hipGetLastError());
-        ok(hipEventRecord(e1, st));
-        float ms = 0.f;
-        if (ok(hipEventSynchronize(e1)) && ok(hipEventElapsedTime(&ms, e0, e1))) *ms_per_pass = ms / reps;
-    }
-    if (st) (void)hipStreamSynchronize(st);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (st) (void)hipStreamDestroy(st);
-    if (a) (void)hipFree(a);
-    if (b) (void)hipFree(b);
-    return rc;
-}
-
-int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy) {
-    return ice_stream_bench(device, bytes, reps, 0, ms_per_copy);
-}
-
 int ice_get_info(IceHandle *h, IceInfo *info) {
     if (!h || !info) return ICE_E_ARG;
     int rc = read_counters(h);

@@ -18,46 +18,6 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *__restrict__ a, size
         a[i] = v;
 }
 
-// Plain device copy: the reference point for HBM bandwidth and for scaling
-// the profiler's FETCH_SIZE / WRITE_SIZE counters.
-// Streaming kernels that measure HBM bandwidth: each thread moves four 16-byte
-// vectors per step, all loads issued before the stores.
-__global__ __launch_bounds__(256) void k_copy4(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n4) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-        b[i] = v0;
-        b[i + stride] = v1;
-        b[i + 2 * stride] = v2;
-        b[i + 3 * stride] = v3;
-    }
-    for (; i < n4; i += stride) b[i] = a[i];
-}
-
-// Read-only: sums the buffer; one word per thread is written only when the
-// sum equals a value the data never produces (never, for finite input).
-__global__ __launch_bounds__(256) void k_read4(const float4 *__restrict__ a, size_t n4, float *__restrict__ sink) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t t = i;
-    float acc = 0.f;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-        acc += (v0.x + v0.y + v0.z + v0.w) + (v1.x + v1.y + v1.z + v1.w) + (v2.x + v2.y + v2.z + v2.w) +
-               (v3.x + v3.y + v3.z + v3.w);
-    }
-    for (; i < n4; i += stride) acc += a[i].x + a[i].y + a[i].z + a[i].w;
-    if (acc == -1.2345e30f) sink[t] = acc;
-}
-
-// Write-only.
-__global__ __launch_bounds__(256) void k_write4(float4 *__restrict__ b, size_t n4) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) b[i] = z;
-}
-
 __global__ void k_step_advance(Counters *cnt, int by) {
     if (threadIdx.x == 0 && blockIdx.x == 0) cnt->step += by;
 }

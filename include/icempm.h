@@ -277,14 +277,6 @@ typedef struct IceInfo {
 int ice_get_info(IceHandle *h, IceInfo *info);                      /* domain 0 */
 int ice_get_domain_info(IceHandle *h, int d, IceInfo *info);
 
-/* Measurement utility (no reference counterpart): average time of a plain
- * device-to-device copy of `bytes` (read + write), over `reps` launches on
- * `device`.  Used to read HBM bandwidth and to scale profiler byte counters. */
-int ice_copy_bench(int device, size_t bytes, int reps, double *ms_per_copy);
-/* The same for mode 0 (copy: read + write `bytes`), 1 (read `bytes`) or 2
- * (write `bytes`). */
-int ice_stream_bench(int device, size_t bytes, int reps, int mode, double *ms_per_pass);
-
 /* ---- learned plastic projection (no reference counterpart; replaces, when
  * deployed, the plastic branch of Wolper_Drucker_Prager, kernels.cuh:431-479).
  * A 4 -> 64 -> 64 -> 3 MLP with SiLU, BF16 weights and activations on MFMA,

@@ -1,6 +1,6 @@
-"""The C-ABI libraries load and export every entry point include/icempm.h and
-include/icempm_io.h declare; the ctypes layouts match the C structs.  No
-compute calls (no GPU)."""
+"""The C-ABI libraries load and export every entry point include/icempm.h,
+include/icempm_io.h and include/icempm_calib.h declare; the ctypes layouts
+match the C structs.  No compute calls (no GPU)."""
 import ctypes as C
 import os
 import re
@@ -43,6 +43,17 @@ def test_io_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(h5io.SIGNATURES)
+
+
+def test_calib_library_exports_every_declared_symbol():
+    names = declared_functions(os.path.join(ROOT, "include", "icempm_calib.h"))
+    lib = abi.load_calib()
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(abi.CALIB_SIGNATURES)
+    ms = C.c_double(1.0)
+    assert lib.ice_stream_bench(0, 4096, 1, 7, C.byref(ms)) == -1 and ms.value == 1.0
+    assert lib.ice_calib_last_error()
 
 
 def test_params_reset_and_derive_without_gpu():
