@@ -320,6 +320,7 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
     __shared__ uint32_t s_rb[RC];
     __shared__ uint32_t s_pre[RC + 1];
     __shared__ int s_next;
+    __shared__ uint32_t s_tally[3];   // the tile's drift, accepted and fallback counts
 
     // Workgroup b takes tile-table entry xcd_remap(b).  PERSIST: the launch is
     // persistent, workgroup b (on XCD b % 8) takes the next entry of that
@@ -405,6 +406,11 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
             }
         }
         for (int k = threadIdx.x; k < 3 * Fr::NN; k += kTileThreads) s_acc[k] = 0.f;
+        if (threadIdx.x < 3) s_tally[threadIdx.x] = 0u;
+        // wave-uniform counts, added to the domain's counters once per tile:
+        // one global atomic per wave on a domain-wide address serialises at
+        // the L2 when most waves count (fast flow, the learned projection)
+        uint32_t w_drift = 0, w_acc = 0, w_fb = 0;
 
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         for (int rd = 0; rd < NROUND; rd++) {
@@ -585,10 +591,8 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                                     }
                                 }
                                 const uint64_t ba = __ballot(accepted), bw = __ballot(want);
-                                if (lane == 0) {
-                                    atomicAdd(&cnt->pad[0], (uint32_t)__popcll(ba));
-                                    atomicAdd(&cnt->pad[1], (uint32_t)__popcll(bw & ~ba));
-                                }
+                                w_acc += (uint32_t)__popcll(ba);
+                                w_fb += (uint32_t)__popcll(bw & ~ba);
                             }
                         }
                         if (act) {
@@ -609,10 +613,8 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                                 stage = true;
                             }
                         }
-                        if (S.complete) {   // points off their sorted cell at a chunk's end (re-sort policy)
-                            const uint64_t bm = __ballot(stage && (s.u & kMoved));
-                            if (lane == 0 && bm) atomicAdd(&cnt->n_drift, (uint32_t)__popcll(bm));
-                        }
+                        if (S.complete)   // points off their sorted cell at a chunk's end (re-sort policy)
+                            w_drift += (uint32_t)__popcll(__ballot(stage && (s.u & kMoved)));
                         stage_terms<!SURR>(s_st, s_cw, jj, stage, q, fx0, fy0);
                     }
                 } else {
@@ -711,6 +713,19 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                 __syncthreads();   // stencils read: the area takes the next staged terms
             }
             __syncthreads();   // the round's ranges are read; the next round may overwrite them
+        }
+        if (S.complete || SURR) {   // uniform
+            if (lane == 0) {
+                if (w_drift) atomicAdd(&s_tally[0], w_drift);
+                if (w_acc) atomicAdd(&s_tally[1], w_acc);
+                if (w_fb) atomicAdd(&s_tally[2], w_fb);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (s_tally[0]) atomicAdd(&cnt->n_drift, s_tally[0]);
+                if (s_tally[1]) atomicAdd(&cnt->pad[0], s_tally[1]);
+                if (s_tally[2]) atomicAdd(&cnt->pad[1], s_tally[2]);
+            }
         }
         // the frame, every node once (coalesced).  A NaN P2G contribution
         // (kernels.cuh:97-98) makes its node's sum NaN, so the sums are checked
