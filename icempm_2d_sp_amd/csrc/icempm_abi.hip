@@ -101,6 +101,13 @@ struct IceHandle {
     uint32_t *s_count = nullptr, *s_key = nullptr, *s_slot = nullptr, *s_dflag = nullptr, *s_drank = nullptr,
              *s_order = nullptr, *s_bsum = nullptr;
     size_t s_bsum_len = 0;
+    size_t s_len = 0;   // entries of s_key, s_slot, s_dflag, s_order (s_drank: one more)
+    // batched sorts: the job table on the device, its pinned host staging and
+    // the event after its last upload (the staging is rewritten only after it)
+    void *sort_tab = nullptr, *sort_tab_host = nullptr;
+    size_t sort_tab_bytes = 0;
+    hipEvent_t sort_tab_ev = nullptr;
+    bool sort_tab_pending = false;
     // tiles
     // band 1: tiles walk down each column (y fastest), the order of the point
     // and cell layout (x * GridY + y), so neighbouring workgroups read
@@ -465,28 +472,47 @@ int scan2_u32(IceHandle *h, const uint32_t *in0, size_t n0, uint32_t *out0, cons
 // Sort domain d's points into cell order; keep disabled points behind (drop =
 // false) or drop them (drop = true).  Leaves the domain's cs valid and its
 // off-tile list empty.
+// Domain d's ranges of the sort scratch: cell counters at its cs_base, the
+// per-point arrays at its pt_base, the disabled ranks (n + 1 entries) at
+// pt_base + d.
+struct SortScratch {
+    uint32_t *count, *key, *slot, *dflag, *drank, *order;
+};
+SortScratch sort_scratch(const IceHandle *h, int d) {
+    const Domain &D = h->dom[d];
+    return SortScratch{h->s_count + D.cs_base, h->s_key + D.pt_base, h->s_slot + D.pt_base,
+                       h->s_dflag + D.pt_base, h->s_drank + D.pt_base + d, h->s_order + D.pt_base};
+}
+// Domain d's sort ranges lie inside the scratch allocations.
+bool sort_ranges_ok(const IceHandle *h, int d) {
+    const Domain &D = h->dom[d];
+    return (size_t)D.pt_base + d + D.n + 1 <= h->s_len + 1 && D.cs_base + D.N + 2 <= h->cs_total && D.n <= D.capacity;
+}
+
 int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
     Domain &D = h->dom[d];
     const uint32_t n = D.n;
     const uint32_t NC = (uint32_t)D.N;
     uint32_t *cs = h->cs + D.cs_base;
+    const SortScratch X = sort_scratch(h, d);
+    if (!sort_ranges_ok(h, d)) { h->err = "sort scratch does not cover the domain"; return ICE_E_STATE; }
     int rc;
     // a sort that did not finish may have left counts anywhere: clear them all
-    if (!h->s_count_clean && (rc = launch_fill_u32(h, h->s_count, h->max_nodes() + 1, 0u))) return rc;
+    if (!h->s_count_clean && (rc = launch_fill_u32(h, h->s_count, h->cs_total, 0u))) return rc;
     h->s_count_clean = false;
     if (n) {
         hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->dlines(d), n, D.gx, D.gy,
-                           NC, h->s_count, h->s_key, h->s_slot, h->s_dflag, h->dcnt(d));
+                           NC, X.count, X.key, X.slot, X.dflag, h->dcnt(d));
         LAUNCHCHK(h, "k_sort_keys");
     }
-    if ((rc = scan2_u32(h, h->s_count, NC, cs, h->s_dflag, n, h->s_drank))) return rc;
+    if ((rc = scan2_u32(h, X.count, NC, cs, X.dflag, n, X.drank))) return rc;
     if (n) {
-        hipLaunchKernelGGL(k_sort_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, NC, h->s_key,
-                           h->s_slot, cs, h->s_drank, h->s_order);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, NC, X.key,
+                           X.slot, cs, X.drank, X.order);
         LAUNCHCHK(h, "k_sort_scatter");
     }
-    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(NC, 256)), dim3(256), 0, h->stream, NC, cs, h->s_order,
-                       h->s_count);
+    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(NC, 256)), dim3(256), 0, h->stream, NC, cs, X.order,
+                       X.count);
     LAUNCHCHK(h, "k_sort_cells");
     h->s_count_clean = true;
     uint32_t n_active = n;   // only a sort that drops the disabled points needs the count on the host
@@ -500,7 +526,7 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
         const uint64_t to = h->pt_off(d, D.half ^ 1);
         Lines dst{h->pts + to, h->pitch};
         hipLaunchKernelGGL(k_sort_gather, dim3(blocks_for(n_out, 256)), dim3(256), 0, h->stream, h->dlines(d), dst,
-                           h->nlines, n_out, h->s_order, perm_dev);
+                           h->nlines, n_out, X.order, perm_dev);
         LAUNCHCHK(h, "k_sort_gather");
         D.half ^= 1;
     }
@@ -520,6 +546,141 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
     D.steps_since_sort = 0;
     D.n_sorts++;
     return ICE_OK;
+}
+
+// The sorts of domains ds (disabled points kept) as one set of launches:
+// keys, the two scans of every domain, scatter, per-cell ordering, gather and
+// the table entries, each launch over all the domains' blocks.  Each domain
+// ends exactly as device_sort(h, d, false, nullptr) leaves it.  A single
+// domain, or the naive path, goes through device_sort.
+int device_sort_many(IceHandle *h, const std::vector<int> &ds) {
+    if (ds.empty()) return ICE_OK;
+    if (ds.size() == 1 || h->naive()) {
+        for (int d : ds) {
+            int rc = device_sort(h, d, false, nullptr);
+            if (rc) return rc;
+        }
+        return ICE_OK;
+    }
+    const int nj = (int)ds.size();
+    // table layout: SortJob[nj] | ScanJob[2 nj] | pblk[nj + 1] | cblk[nj + 1] | sblk[2 nj + 1]
+    const size_t o_scan = round_up(sizeof(SortJob) * nj, 256);
+    const size_t o_pblk = o_scan + round_up(sizeof(ScanJob) * 2 * nj, 256);
+    const size_t o_cblk = o_pblk + round_up(sizeof(uint32_t) * (nj + 1), 256);
+    const size_t o_sblk = o_cblk + round_up(sizeof(uint32_t) * (nj + 1), 256);
+    const size_t bytes = o_sblk + round_up(sizeof(uint32_t) * (2 * nj + 1), 256);
+    if (bytes > h->sort_tab_bytes) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (h->sort_tab) (void)hipFree(h->sort_tab);
+        if (h->sort_tab_host) (void)hipHostFree(h->sort_tab_host);
+        h->sort_tab = h->sort_tab_host = nullptr;
+        h->sort_tab_bytes = 0;
+        h->sort_tab_pending = false;
+        HIPCHK(h, hipMalloc(&h->sort_tab, bytes));
+        HIPCHK(h, hipHostMalloc(&h->sort_tab_host, bytes));
+        if (!h->sort_tab_ev) HIPCHK(h, hipEventCreateWithFlags(&h->sort_tab_ev, hipEventDisableTiming));
+        h->sort_tab_bytes = bytes;
+    }
+    if (h->sort_tab_pending) HIPCHK(h, hipEventSynchronize(h->sort_tab_ev));   // the last upload has left the staging
+    h->sort_tab_pending = false;
+    char *hb = (char *)h->sort_tab_host, *db = (char *)h->sort_tab;
+    SortJob *job = (SortJob *)hb;
+    ScanJob *sj = (ScanJob *)(hb + o_scan);
+    uint32_t *pblk = (uint32_t *)(hb + o_pblk), *cblk = (uint32_t *)(hb + o_cblk), *sblk = (uint32_t *)(hb + o_sblk);
+    size_t np = 0, nc = 0, ns = 0, bs = 0;
+    for (int j = 0; j < nj; j++) {
+        const int d = ds[j];
+        const Domain &D = h->dom[d];
+        if (!sort_ranges_ok(h, d)) { h->err = "sort scratch does not cover the domain"; return ICE_E_STATE; }
+        const SortScratch X = sort_scratch(h, d);
+        const uint64_t to = h->pt_off(d, D.half ^ 1);
+        SortJob &J = job[j];
+        J.src = h->dlines(d);
+        J.dst = Lines{h->pts + to, h->pitch};
+        J.n = D.n;
+        J.NC = (uint32_t)D.N;
+        J.gx = D.gx;
+        J.gy = D.gy;
+        J.count = X.count;
+        J.cs = h->cs + D.cs_base;
+        J.key = X.key;
+        J.slot = X.slot;
+        J.dflag = X.dflag;
+        J.drank = X.drank;
+        J.order = X.order;
+        J.cnt = h->dcnt(d);
+        J.entry = (D.n && h->d_desc && !h->tables_dirty) ? h->d_desc + d : nullptr;
+        J.pt_off = D.n ? to : h->pt_off(d, D.half);
+        pblk[j] = (uint32_t)np;
+        cblk[j] = (uint32_t)nc;
+        np += (D.n + 255) / 256;
+        nc += (D.N + 255) / 256;
+        // the two scans of the domain: cell counts -> cs, disabled flags -> ranks
+        const size_t len[2] = {D.N, D.n};
+        const uint32_t *in[2] = {J.count, J.dflag};
+        uint32_t *out[2] = {J.cs, J.drank};
+        for (int k = 0; k < 2; k++) {
+            const uint32_t nb = (uint32_t)std::max<size_t>(1, (len[k] + kScanTile - 1) / kScanTile);
+            if (bs + nb + 1 > 2 * h->s_bsum_len) { h->err = "scan scratch too small"; return ICE_E_STATE; }
+            sj[2 * j + k] = ScanJob{in[k], len[k], h->s_bsum + bs, nb, out[k]};
+            sblk[2 * j + k] = (uint32_t)ns;
+            ns += nb;
+            bs += nb + 1;
+        }
+    }
+    pblk[nj] = (uint32_t)np;
+    cblk[nj] = (uint32_t)nc;
+    sblk[2 * nj] = (uint32_t)ns;
+    if (np >= (1u << 31) || nc >= (1u << 31) || ns >= (1u << 31)) { h->err = "sort batch too large"; return ICE_E_STATE; }
+    HIPCHK(h, hipMemcpyAsync(db, hb, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipEventRecord(h->sort_tab_ev, h->stream));
+    h->sort_tab_pending = true;
+    const SortJob *djob = (const SortJob *)db;
+    const ScanJob *dsj = (const ScanJob *)(db + o_scan);
+    const uint32_t *dp = (const uint32_t *)(db + o_pblk), *dc = (const uint32_t *)(db + o_cblk),
+                   *dsb = (const uint32_t *)(db + o_sblk);
+    int rc;
+    if (!h->s_count_clean && (rc = launch_fill_u32(h, h->s_count, h->cs_total, 0u))) return rc;
+    h->s_count_clean = false;
+    if (np) {
+        hipLaunchKernelGGL(k_sort_keys_many, dim3((unsigned)np), dim3(256), 0, h->stream, djob, dp, nj);
+        LAUNCHCHK(h, "k_sort_keys_many");
+    }
+    hipLaunchKernelGGL(k_scan_reduce_many, dim3((unsigned)ns), dim3(kScanThreads), 0, h->stream, dsj, dsb, 2 * nj);
+    LAUNCHCHK(h, "k_scan_reduce_many");
+    hipLaunchKernelGGL(k_scan_blocks_many, dim3((unsigned)(2 * nj)), dim3(kScanThreads), 0, h->stream, dsj);
+    LAUNCHCHK(h, "k_scan_blocks_many");
+    hipLaunchKernelGGL(k_scan_apply_many, dim3((unsigned)ns), dim3(kScanThreads), 0, h->stream, dsj, dsb, 2 * nj);
+    LAUNCHCHK(h, "k_scan_apply_many");
+    if (np) {
+        hipLaunchKernelGGL(k_sort_scatter_many, dim3((unsigned)np), dim3(256), 0, h->stream, djob, dp, nj);
+        LAUNCHCHK(h, "k_sort_scatter_many");
+    }
+    hipLaunchKernelGGL(k_sort_cells_many, dim3((unsigned)nc), dim3(256), 0, h->stream, djob, dc, nj);
+    LAUNCHCHK(h, "k_sort_cells_many");
+    h->s_count_clean = true;
+    if (np) {
+        hipLaunchKernelGGL(k_sort_gather_many, dim3((unsigned)np), dim3(256), 0, h->stream, djob, dp, nj, h->nlines);
+        LAUNCHCHK(h, "k_sort_gather_many");
+    }
+    hipLaunchKernelGGL(k_sort_done_many, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, h->stream, djob, nj);
+    LAUNCHCHK(h, "k_sort_done_many");
+    for (int j = 0; j < nj; j++) {
+        Domain &D = h->dom[ds[j]];
+        if (D.n) D.half ^= 1;
+        D.sorted = true;
+        D.steps_since_sort = 0;
+        D.n_sorts++;
+    }
+    return ICE_OK;
+}
+
+// Sort every domain not in cell order (one batch of sorts).
+int sort_unsorted(IceHandle *h) {
+    std::vector<int> ds;
+    for (int d = 0; d < h->ndom(); d++)
+        if (!h->dom[d].sorted) ds.push_back(d);
+    return device_sort_many(h, ds);
 }
 
 TimingSet *cur_timing(IceHandle *h) {
@@ -601,12 +762,9 @@ int phase_p2g(IceHandle *h) {
         rec(h, 2);
         return ICE_OK;
     }
-    for (int d = 0; d < h->ndom(); d++)
-        if (!h->dom[d].sorted) {
-            int rc = device_sort(h, d, false, nullptr);
-            if (rc) return rc;
-        }
-    int rc = upload_tables(h);
+    int rc = sort_unsorted(h);
+    if (rc) return rc;
+    rc = upload_tables(h);
     if (rc) return rc;
     const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
     const GridLines G = grid_lines(h);
@@ -762,8 +920,7 @@ int pipeline_fill(IceHandle *h) {
 int phase_pipelined(IceHandle *h, const StepArg &S) {
     if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
     int rc;
-    for (int d = 0; d < h->ndom(); d++)
-        if (!h->dom[d].sorted && (rc = device_sort(h, d, false, nullptr))) return rc;
+    if ((rc = sort_unsorted(h))) return rc;
     if ((rc = upload_tables(h))) return rc;
     if (!h->frames_valid && (rc = pipeline_fill(h))) return rc;
     rec(h, 1);
@@ -802,8 +959,7 @@ int phase_pipelined(IceHandle *h, const StepArg &S) {
 int phase_p2g_pipelined(IceHandle *h) {
     if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
     int rc;
-    for (int d = 0; d < h->ndom(); d++)
-        if (!h->dom[d].sorted && (rc = device_sort(h, d, false, nullptr))) return rc;
+    if ((rc = sort_unsorted(h))) return rc;
     if ((rc = upload_tables(h))) return rc;
     if (!h->frames_valid && (rc = pipeline_fill(h))) return rc;
     h->p2g_called = true;
@@ -911,12 +1067,12 @@ int alloc_grid(IceHandle *h) {
     h->total_nodes = std::max<size_t>(nb, 64);
     h->gpitch = h->total_nodes;
     h->cs_total = std::max<size_t>(cb, 64);
-    const size_t maxN = h->max_nodes();
     HIPCHK(h, hipMalloc(&h->grid, sizeof(float) * h->gpitch * kGridLines));
     HIPCHK(h, hipMalloc(&h->status, round_up(h->total_nodes, 256)));
     HIPCHK(h, hipMalloc(&h->cnt, sizeof(Counters) * (h->dom.size() + 1)));
     HIPCHK(h, hipHostMalloc(&h->host_cnt, sizeof(Counters) * (h->dom.size() + 1)));
-    HIPCHK(h, hipMalloc(&h->s_count, sizeof(uint32_t) * (maxN + 1)));
+    HIPCHK(h, hipMalloc(&h->s_count, sizeof(uint32_t) * h->cs_total));   // each domain's counters at its cs_base
+    HIPCHK(h, hipMemsetAsync(h->s_count, 0, sizeof(uint32_t) * h->cs_total, h->stream));
     HIPCHK(h, hipMalloc(&h->cs, sizeof(uint32_t) * h->cs_total));
     HIPCHK(h, hipMemsetAsync(h->grid, 0, sizeof(float) * h->gpitch * kGridLines, h->stream));
     HIPCHK(h, hipMemsetAsync(h->status, ICE_GRID_STATUS_WATER, round_up(h->total_nodes, 256), h->stream));
@@ -954,7 +1110,10 @@ int alloc_points(IceHandle *h) {
     h->pitch = base;
     h->half_floats = h->pitch * ICE_N_LINES_MAX;
     const size_t bytes = sizeof(float) * h->half_floats;
-    const size_t mc = h->max_capacity();
+    // sort scratch: every domain's range at its pt_base (drank: pt_base + d,
+    // one more entry per domain), so many domains sort in one set of launches
+    const size_t mc = h->total_capacity + h->dom.size();
+    h->s_len = mc;
     HIPCHK(h, hipMalloc(&h->pts, 2 * bytes));
     HIPCHK(h, hipMemsetAsync(h->pts, 0, bytes, h->stream));   // half 0, where every domain starts
     HIPCHK(h, hipMalloc(&h->offtile, sizeof(uint32_t) * h->total_capacity));
@@ -963,8 +1122,12 @@ int alloc_points(IceHandle *h) {
     HIPCHK(h, hipMalloc(&h->s_dflag, sizeof(uint32_t) * mc));
     HIPCHK(h, hipMalloc(&h->s_drank, sizeof(uint32_t) * (mc + 1)));
     HIPCHK(h, hipMalloc(&h->s_order, sizeof(uint32_t) * mc));
-    h->s_bsum_len = std::max<size_t>(mc, h->max_nodes()) / kScanTile + 8;
-    HIPCHK(h, hipMalloc(&h->s_bsum, sizeof(uint32_t) * 2 * h->s_bsum_len));   // one half per scan of a pass
+    // block sums: one half per scan of a single sort's pass; a batch of sorts
+    // lays out nb + 1 per scan job over both halves (at most the total points
+    // plus the total cells over kScanTile, plus 2 per job and domain)
+    h->s_bsum_len = std::max<size_t>((mc + h->cs_total) / kScanTile + 4 * h->dom.size(),
+                                     std::max<size_t>(h->max_capacity(), h->max_nodes()) / kScanTile) + 8;
+    HIPCHK(h, hipMalloc(&h->s_bsum, sizeof(uint32_t) * 2 * h->s_bsum_len));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->tables_dirty = true;
     return ICE_OK;
@@ -1230,6 +1393,9 @@ int ice_destroy(IceHandle *h) {
                     (void *)h->mlp_rec_cnt})
         if (p) (void)hipFree(p);
     if (h->host_cnt) (void)hipHostFree(h->host_cnt);
+    if (h->sort_tab) (void)hipFree(h->sort_tab);
+    if (h->sort_tab_host) (void)hipHostFree(h->sort_tab_host);
+    if (h->sort_tab_ev) (void)hipEventDestroy(h->sort_tab_ev);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return ICE_OK;
@@ -1421,12 +1587,10 @@ int ice_substep(IceHandle *h, double t, int recordPQ) {
 int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
     if (!h || nsteps < 0) return ICE_E_ARG;
     HIPCHK(h, hipSetDevice(h->device));
-    if (!h->naive())
-        for (int d = 0; d < h->ndom(); d++)
-            if (!h->dom[d].sorted) {
-                int rc = device_sort(h, d, false, nullptr);
-                if (rc) return rc;
-            }
+    if (!h->naive()) {
+        int rc = sort_unsorted(h);
+        if (rc) return rc;
+    }
     int rc = h->naive() ? ICE_OK : upload_tables(h);
     if (rc) return rc;
     if (h->pipelined() && !h->frames_valid && (rc = pipeline_fill(h))) return rc;   // never inside a graph
@@ -1461,13 +1625,15 @@ int ice_run(IceHandle *h, int first_step, int nsteps, int use_graph) {
             // (pipelined path) those where resort_drift % of the points sit off
             // their sorted cell: those escape in bulk within the next chunk
             if ((rc = read_counters(h))) return rc;
+            std::vector<int> ds;
             for (int d = 0; d < h->ndom(); d++) {
                 const Domain &D = h->dom[d];
                 const uint32_t limit = std::max<uint32_t>(256, D.n / 256);
                 const bool drifted = h->resort_drift > 0 &&
                                      (uint64_t)h->host_cnt[d].n_drift * 100 > (uint64_t)D.n * h->resort_drift;
-                if ((h->host_cnt[d].n_esc > limit || drifted) && (rc = device_sort(h, d, false, nullptr))) return rc;
+                if (h->host_cnt[d].n_esc > limit || drifted) ds.push_back(d);
             }
+            if ((rc = device_sort_many(h, ds))) return rc;
         }
     }
     h->last_t = h->prm.InitialTimeStep * (first_step + nsteps - 1);
@@ -1674,8 +1840,7 @@ int ice_autotune(IceHandle *h, int nsteps, int *tile_w, int *tile_h, double *bes
     if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
     HIPCHK(h, hipSetDevice(h->device));
     int rc;
-    for (int d = 0; d < h->ndom(); d++)
-        if (!h->dom[d].sorted && (rc = device_sort(h, d, false, nullptr))) return rc;
+    if ((rc = sort_unsorted(h))) return rc;
     // Save everything a trial changes (points in both halves of the point
     // buffer, cell starts, force lines, counters, off-tile lists) and restore
     // it, with the domains' halves, after each candidate, so tuning leaves the

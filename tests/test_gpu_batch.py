@@ -126,3 +126,50 @@ def test_batch_det_with_offtile_lists_equals_single_domains_bitwise():
         assert (eb, db) == (es, ds)
         s.close()
     b.close()
+
+
+def test_batch_resorts_match_single_domain_sorts():
+    """The re-sorts between run chunks: a batch sorts every domain that needs
+    it in one set of launches (keys, scans, scatter, cell order, gather over
+    all those domains' blocks).  Each domain comes out in the same device
+    order, bit for bit, and after as many sorts as the same domain run alone,
+    whose sorts go through the single-domain path.  Deterministic mode, so
+    the states between the sorts are bit-identical too; the pipelined mode
+    also re-sorts on drift, and its batch still matches the oracle."""
+    print("test_batch_resorts_match_single_domain_sorts", flush=True)
+    doms = _domains()
+    flags = doms[0].params.flags | abi.FLAG_DETERMINISTIC
+    b = BatchSimulation(doms, flags=flags)
+    b.run(0, 64, graph=True)
+    sorts = [b.info(d).n_sorts for d in range(len(doms))]
+    print("sorts per domain", sorts, flush=True)
+    assert sum(1 for v in sorts if v >= 2) >= 2   # the first sort (all domains at once) and re-sorts
+    for d, sc in enumerate(doms):
+        s = Simulation(sc, flags=flags)
+        s.run(0, 64, graph=True)
+        assert s.info().n_sorts == sorts[d], d
+        ps, fs, es, ds = s.download()
+        pb, fb, eb, db = b.download(d)
+        np.testing.assert_array_equal(pb.view(np.uint32), ps.view(np.uint32))   # device order
+        np.testing.assert_array_equal(fb.view(np.uint32), fs.view(np.uint32))
+        assert (eb, db) == (es, ds)
+        s.close()
+    b.close()
+
+    p = BatchSimulation(doms, options={"resort_drift": 30})
+    p.run(0, 32, graph=True)
+    sorts = [p.info(d).n_sorts for d in range(len(doms))]
+    print("pipelined sorts per domain", sorts, flush=True)
+    assert sum(1 for v in sorts if v > 1) >= 2
+    for d, sc in enumerate(doms):
+        pts, _, err, dis = p.download(d)
+        o = oracle.Oracle.from_scene(sc, 64)
+        o.run(0, 32)
+        compare_points(pts, o.points(), fields=[
+            ("pos", [abi.IDX_POSX, abi.IDX_POSX + 1], 1e-4),
+            ("vel", [abi.IDX_VELX, abi.IDX_VELX + 1], 1e-4),
+            ("Fe", list(range(abi.IDX_FE00, abi.IDX_FE00 + 4)), 1e-4),
+        ], max_bad_frac=0.01, max_flag_mismatch=0.01)
+        assert err == o.error and dis == o.disabled
+        o.close()
+    p.close()
