@@ -313,7 +313,7 @@ using ToGridFn = void (*)(const DomainDesc *, int, const float *, GridLines);
 
 struct TileKernels {
     int w, h;
-    P2GFn p2g, p2g_det;
+    P2GFn p2g_det;   // deterministic mode's P2G (fixed point)
     G2PFn g2p;
     PipeFn fill, fused, fused_surr, fill_persistent, fused_persistent;
     PipeFn fused_buf, fused_surr_buf;   // 32-bit addressing (BufLines, 32-bit frame offsets)
@@ -322,7 +322,7 @@ struct TileKernels {
 };
 
 #define TK(W, H)                                                                                          \
-    {W, H, k_p2g_tiles<W, H, false>, k_p2g_tiles<W, H, true>, k_g2p_tiles<W, H>, k_tile_pipeline<W, H, false>, \
+    {W, H, k_p2g_tiles_det<W, H>, k_g2p_tiles<W, H>, k_tile_pipeline<W, H, false>,                          \
      k_tile_pipeline<W, H, true>, k_tile_pipeline<W, H, true, true>, k_tile_pipeline<W, H, false, false, true>, \
      k_tile_pipeline<W, H, true, false, true>, k_tile_pipeline<W, H, true, false, false, true>,             \
      k_tile_pipeline<W, H, true, true, false, true>, k_offtile_pipeline<W, H>, k_frames_to_grid<W, H>}
@@ -741,6 +741,8 @@ DevParams dom_params(const IceHandle *h, int d) {
     return P;
 }
 
+int tile_launch_shape(IceHandle *h, PipeFn &fn, PipeFn persistent_fn, unsigned &grid, uint32_t *&queue);
+
 // Phases of one substep.  Event slots: 0 start, 1 after the main P2G kernel,
 // 2 after P2G, 3 before the main G2P kernel, 4 after G2P.
 int phase_p2g(IceHandle *h) {
@@ -768,19 +770,39 @@ int phase_p2g(IceHandle *h) {
     if (rc) return rc;
     const TileKernels *tk = find_tile(h->tile_w, h->tile_h);
     const GridLines G = grid_lines(h);
+    if (h->det()) {
+        // fixed-point pull per tile; the tile kernel also adds the off-tile points
+        if (h->ntiles) {
+            hipLaunchKernelGGL(tk->p2g_det, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp, h->d_desc,
+                               h->d_tiles, h->ntiles, h->lines(), h->cs, G, h->cnt, h->offtile, h->fix);
+            LAUNCHCHK(h, "k_p2g_tiles_det");
+        }
+        rec(h, 1);
+        rec(h, 2);
+        return ICE_OK;
+    }
+    // the tile pass's P2G (points loaded one per lane, their terms staged in
+    // LDS for the cell lanes: each point line read once) into the frames, the
+    // off-tile points into the same frames, then every node's frame sum into
+    // the grid lines
+    float *F = h->fr[h->fr_cur];
+    const StepArg S0{0.0, 0, nullptr, 0};
     if (h->ntiles) {
-        hipLaunchKernelGGL(h->det() ? tk->p2g_det : tk->p2g, dim3(h->ntiles), dim3(kTileThreads), 0, h->stream, h->dp,
-                           h->d_desc, h->d_tiles, h->ntiles, h->lines(), h->cs, G, h->cnt, h->offtile, h->fix);
-        LAUNCHCHK(h, "k_p2g_tiles");
+        unsigned grid;
+        uint32_t *queue;
+        PipeFn fn = tk->fill;
+        if ((rc = tile_launch_shape(h, fn, tk->fill_persistent, grid, queue))) return rc;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(kTileThreads), 0, h->stream, h->dp, S0, SurrArg{}, h->d_desc,
+                           h->d_tiles, h->ntiles, h->lines(), (const uint32_t *)h->cs, G, h->cnt, h->offtile,
+                           (const float *)F, F, queue);
+        LAUNCHCHK(h, "k_tile_pipeline (P2G)");
     }
     rec(h, 1);
-    if (!h->det()) {   // the deterministic tile kernel adds the off-tile points itself
-        const uint32_t *pre;
-        if ((rc = offtile_prefix(h, pre))) return rc;
-        hipLaunchKernelGGL(k_p2g_offtile, dim3(offtile_blocks(h)), dim3(256), 0, h->stream, h->dp,
-                           (const DomainDesc *)h->d_desc, h->ndom(), pre, h->lines(), G, h->cnt,
-                           (const uint32_t *)h->offtile);
-        LAUNCHCHK(h, "k_p2g_offtile");
+    if ((rc = launch_offtile(h, tk, S0, G, F, F, 0))) return rc;
+    for (int d = 0; d < h->ndom(); d++) {
+        hipLaunchKernelGGL(tk->to_grid, dim3((unsigned)std::min<size_t>(blocks_for(h->dom[d].N, 256), 1024)),
+                           dim3(256), 0, h->stream, (const DomainDesc *)h->d_desc, d, (const float *)F, G);
+        LAUNCHCHK(h, "k_frames_to_grid");
     }
     rec(h, 2);
     return ICE_OK;

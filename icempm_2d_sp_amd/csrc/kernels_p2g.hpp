@@ -1,22 +1,18 @@
-// kernels_p2g.hpp — the sorted path's P2G: one workgroup per tile of TW x TH
-// nodes, pulling the points sorted into cells T +- 2, one lane per cell.
+// kernels_p2g.hpp — the deterministic mode's P2G: one workgroup per tile of
+// TW x TH nodes, pulling the points sorted into cells T +- 2, one lane per
+// cell (simulation/kernels.cuh:39-104).
 //
-// A lane walks the points sorted into its cell and sums their 3x3 stencil
-// contributions (simulation/kernels.cuh:39-104) in registers; unmoved points
-// need no cell-word load.  By default the workgroup then adds the cell
-// stencils in nine barrier-separated phases of plain LDS read-add-write, one
-// per stencil offset (within a phase all cells of a round hit distinct nodes),
-// so no LDS atomics remain on the common path and the float sum order is
-// fixed.  Points that changed cell since the sort add their own stencil with
-// LDS atomics in a phase of their own.  Every node of the tile is then stored
-// once: no global atomics and no grid clear.
-//
-// Deterministic mode: every contribution is converted to int64 fixed point
-// on its own and the sums are integer (independent of order and of how points
-// are grouped into cells), and the tile kernel also adds the domain's
-// off-tile points, so results are bit-identical for any schedule and any sort
-// history.  Otherwise k_p2g_offtile adds the off-tile
-// points with float global atomics after the tiles.
+// Every contribution is converted to int64 fixed point on its own and the
+// sums are integer (independent of order and of how points are grouped into
+// cells): a lane walks the points sorted into its cell and sums their 3x3
+// stencil contributions in registers (unmoved points need no cell-word load),
+// then adds the stencil to the tile's nodes with LDS integer atomics; points
+// that changed cell since the sort add their own stencil, and the tile kernel
+// also adds the domain's off-tile points, so results are bit-identical for
+// any schedule and any sort history.  Every node of the tile is then stored
+// once: no global atomics and no grid clear.  (The float P2G of the other
+// sorted paths is the tile pass's: points loaded one per lane and their terms
+// staged in LDS for the cell lanes, kernels_fused.hpp.)
 #pragma once
 #include "tile_common.hpp"
 
@@ -28,24 +24,17 @@ __device__ __forceinline__ long long to_fix(float v, float scale) {
     return __builtin_isnan(v) ? 0ll : (long long)(v * scale);
 }
 
-template <bool DET, int NN>
-__device__ __forceinline__ void lds_add3(typename std::conditional<DET, unsigned long long, float>::type *acc, int o,
-                                         float dm, float dpx, float dpy, const FixScale &fs) {
-    if constexpr (DET) {
-        atomicAdd(&acc[o], (unsigned long long)to_fix(dm, fs.m));
-        atomicAdd(&acc[NN + o], (unsigned long long)to_fix(dpx, fs.p));
-        atomicAdd(&acc[2 * NN + o], (unsigned long long)to_fix(dpy, fs.p));
-    } else {
-        __hip_atomic_fetch_add(&acc[o], dm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&acc[NN + o], dpx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&acc[2 * NN + o], dpy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+template <int NN>
+__device__ __forceinline__ void lds_add3(unsigned long long *acc, int o, float dm, float dpx, float dpy,
+                                         const FixScale &fs) {
+    atomicAdd(&acc[o], (unsigned long long)to_fix(dm, fs.m));
+    atomicAdd(&acc[NN + o], (unsigned long long)to_fix(dpx, fs.p));
+    atomicAdd(&acc[2 * NN + o], (unsigned long long)to_fix(dpy, fs.p));
 }
 
 // Add one point's whole stencil (cell (cx, cy)) to the tile's nodes.
-template <int TW, int TH, int LP, bool DET>
-__device__ __forceinline__ void scatter_point(const P2GPoint &q, int x0, int y0,
-                                              typename std::conditional<DET, unsigned long long, float>::type *acc,
+template <int TW, int TH, int LP>
+__device__ __forceinline__ void scatter_point(const P2GPoint &q, int x0, int y0, unsigned long long *acc,
                                               const FixScale &fs, uint32_t &err) {
     constexpr int NN = TW * LP;
     float wx[3], wy[3];
@@ -61,21 +50,16 @@ __device__ __forceinline__ void scatter_point(const P2GPoint &q, int x0, int y0,
             if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
             const int lx = q.cx + a - x0, ly = q.cy + b - y0;
             if ((unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH)
-                lds_add3<DET, NN>(acc, lx * LP + ly, dm, dpx, dpy, fs);
+                lds_add3<NN>(acc, lx * LP + ly, dm, dpx, dpy, fs);
         }
 }
 
-template <bool DET>
-using StencilAcc = typename std::conditional<DET, long long, float>::type;
-
-// Sum the unmoved points of cell (x, y) into the register stencil (DET: each
-// contribution in fixed point); returns whether any point contributed and
-// sets *moved when the cell also holds points that changed cell since the sort.
-template <bool DET>
+// Sum the unmoved points of cell (x, y) into the register stencil, each
+// contribution in fixed point; returns whether any point contributed and sets
+// *moved when the cell also holds points that changed cell since the sort.
 __device__ __forceinline__ bool cell_stencil(const Lines &L, const DevParams &P, uint32_t b, uint32_t e, int x, int y,
-                                             bool ring2, StencilAcc<DET> am[3][3], StencilAcc<DET> ax[3][3],
-                                             StencilAcc<DET> ay[3][3], bool &moved, uint32_t &err,
-                                             const FixScale &fs) {
+                                             bool ring2, long long am[3][3], long long ax[3][3], long long ay[3][3],
+                                             bool &moved, uint32_t &err, const FixScale &fs) {
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -100,39 +84,22 @@ __device__ __forceinline__ bool cell_stencil(const Lines &L, const DevParams &P,
             for (int bb = 0; bb < 3; bb++) {
                 float dm, dpx, dpy;
                 p2g_contrib(q, a - 1, bb - 1, wx[a] * wy[bb], dm, dpx, dpy);
-                if constexpr (DET) {   // the conversion scatter_point's contributions take
-                    if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-                    if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
-                    am[a][bb] += to_fix(dm, fs.m);
-                    ax[a][bb] += to_fix(dpx, fs.p);
-                    ay[a][bb] += to_fix(dpy, fs.p);
-                } else {
-                    am[a][bb] += dm;
-                    ax[a][bb] += dpx;
-                    ay[a][bb] += dpy;
-                }
+                // the conversion scatter_point's contributions take
+                if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
+                if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
+                am[a][bb] += to_fix(dm, fs.m);
+                ax[a][bb] += to_fix(dpx, fs.p);
+                ay[a][bb] += to_fix(dpy, fs.p);
             }
         any = true;
-    }
-    if constexpr (!DET) {
-        if (any) {
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int bb = 0; bb < 3; bb++) {
-                    if (__builtin_isnan(ax[a][bb]) || __builtin_isnan(ay[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-                    if (__builtin_isnan(am[a][bb])) err |= ICE_ERR_GRID_P2G_NAN_MASS;
-                }
-        }
     }
     return any;
 }
 
 // Moved points of cell [b, e): their own stencil, added with LDS atomics.
-template <int TW, int TH, int LP, bool DET>
+template <int TW, int TH, int LP>
 __device__ __forceinline__ void scatter_moved(const Lines &L, const DevParams &P, uint32_t b, uint32_t e, int x0, int y0,
-                                              typename std::conditional<DET, unsigned long long, float>::type *acc,
-                                              const FixScale &fs, uint32_t &err) {
+                                              unsigned long long *acc, const FixScale &fs, uint32_t &err) {
     for (uint32_t i = b; i < e; i++) {
         const uint32_t u = L.u(ICE_IDX_UTILITY, i);
         if ((u & (kDisabled | kEscaped)) || !(u & kMoved)) continue;
@@ -140,12 +107,12 @@ __device__ __forceinline__ void scatter_moved(const Lines &L, const DevParams &P
         P2GPoint q;
         load_p2g_point(L, i, cw, P, q);
         if (!stencil_inside(q.cx, q.cy, P)) { err |= ICE_ERR_POINT_LEFT_AREA; continue; }
-        scatter_point<TW, TH, LP, DET>(q, x0, y0, acc, fs, err);
+        scatter_point<TW, TH, LP>(q, x0, y0, acc, fs, err);
     }
 }
 
-template <int TW, int TH, bool DET>
-__global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const DomainDesc *__restrict__ D,
+template <int TW, int TH>
+__global__ __launch_bounds__(kTileThreads) void k_p2g_tiles_det(DevParams P0, const DomainDesc *__restrict__ D,
                                                             const TileRef *__restrict__ T, int ntiles, Lines L0,
                                                             const uint32_t *__restrict__ cs0, GridLines G0,
                                                             Counters *cnt0, uint32_t *off0, FixScale fs) {
@@ -160,8 +127,7 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const 
     constexpr int NCELL = RW * RH;
     constexpr int LP = TH + 1;                // padded LDS column pitch
     constexpr int NN = TW * LP;
-    using Acc = typename std::conditional<DET, unsigned long long, float>::type;
-    __shared__ Acc s_acc[3 * NN];
+    __shared__ unsigned long long s_acc[3 * NN];
 
     for (int k = threadIdx.x; k < 3 * NN; k += kTileThreads) s_acc[k] = 0;
     __syncthreads();
@@ -181,47 +147,26 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const 
             e = cs[cell + 1];
         }
         const bool ring2 = (c == 0 || c == RW - 1 || r == 0 || r == RH - 1);
-        StencilAcc<DET> am[3][3], ax[3][3], ay[3][3];
+        long long am[3][3], ax[3][3], ay[3][3];
         bool moved = false;
-        const bool any = valid && cell_stencil<DET>(L, P, b, e, x, y, ring2, am, ax, ay, moved, err, fs);
-        if constexpr (DET) {
-            if (any) {
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-#pragma unroll
-                    for (int bb = 0; bb < 3; bb++) {
-                        const int lx = x + a - 1 - x0, ly = y + bb - 1 - y0;
-                        if ((unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH) {
-                            const int o = lx * LP + ly;
-                            atomicAdd(&s_acc[o], (unsigned long long)am[a][bb]);
-                            atomicAdd(&s_acc[NN + o], (unsigned long long)ax[a][bb]);
-                            atomicAdd(&s_acc[2 * NN + o], (unsigned long long)ay[a][bb]);
-                        }
-                    }
-            }
-            if (moved) scatter_moved<TW, TH, LP, DET>(L, P, b, e, x0, y0, s_acc, fs, err);
-        } else {
-            // Nine phases, one per stencil offset: inside a phase every cell of
-            // the round targets a different node, so plain LDS read-add-write
-            // suffices, and the sum order is fixed.
+        const bool any = valid && cell_stencil(L, P, b, e, x, y, ring2, am, ax, ay, moved, err, fs);
+        if (any) {
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int bb = 0; bb < 3; bb++) {
-                    __syncthreads();
                     const int lx = x + a - 1 - x0, ly = y + bb - 1 - y0;
-                    if (any && (unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH) {
+                    if ((unsigned)lx < (unsigned)TW && (unsigned)ly < (unsigned)TH) {
                         const int o = lx * LP + ly;
-                        s_acc[o] += am[a][bb];
-                        s_acc[NN + o] += ax[a][bb];
-                        s_acc[2 * NN + o] += ay[a][bb];
+                        atomicAdd(&s_acc[o], (unsigned long long)am[a][bb]);
+                        atomicAdd(&s_acc[NN + o], (unsigned long long)ax[a][bb]);
+                        atomicAdd(&s_acc[2 * NN + o], (unsigned long long)ay[a][bb]);
                     }
                 }
-            __syncthreads();
-            if (moved) scatter_moved<TW, TH, LP, DET>(L, P, b, e, x0, y0, s_acc, fs, err);
         }
+        if (moved) scatter_moved<TW, TH, LP>(L, P, b, e, x0, y0, s_acc, fs, err);
     }
-    if constexpr (DET) {
+    {
         // off-tile points, order-independent in fixed point
         const uint32_t n_off = cnt->n_esc;
         for (uint32_t j = threadIdx.x; j < n_off; j += kTileThreads) {
@@ -234,7 +179,7 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const 
             P2GPoint q;
             load_p2g_point(L, i, cw, P, q);
             if (!stencil_inside(q.cx, q.cy, P)) { err |= ICE_ERR_POINT_LEFT_AREA; continue; }
-            scatter_point<TW, TH, LP, DET>(q, x0, y0, s_acc, fs, err);
+            scatter_point<TW, TH, LP>(q, x0, y0, s_acc, fs, err);
         }
     }
     __syncthreads();
@@ -244,60 +189,12 @@ __global__ __launch_bounds__(kTileThreads) void k_p2g_tiles(DevParams P0, const 
         const int o = lx * LP + ly;
         if (x < P.gx && y < P.gy) {
             const size_t node = (size_t)x * P.gy + y;
-            if constexpr (DET) {
-                tc.G.m[node] = (float)((double)(long long)s_acc[o] * fs.inv_m);
-                tc.G.px[node] = (float)((double)(long long)s_acc[NN + o] * fs.inv_p);
-                tc.G.py[node] = (float)((double)(long long)s_acc[2 * NN + o] * fs.inv_p);
-            } else {
-                tc.G.m[node] = s_acc[o];
-                tc.G.px[node] = s_acc[NN + o];
-                tc.G.py[node] = s_acc[2 * NN + o];
-            }
+            tc.G.m[node] = (float)((double)(long long)s_acc[o] * fs.inv_m);
+            tc.G.px[node] = (float)((double)(long long)s_acc[NN + o] * fs.inv_p);
+            tc.G.py[node] = (float)((double)(long long)s_acc[2 * NN + o] * fs.inv_p);
         }
     }
     flush_err(cnt, err);
-}
-
-// P2G of the off-tile points: global atomics on top of the tile stores.  All
-// domains' lists as one item space (offtile_domain).
-__global__ __launch_bounds__(256) void k_p2g_offtile(DevParams P0, const DomainDesc *__restrict__ D, int ndom,
-                                                     const uint32_t *__restrict__ pre, Lines L0, GridLines G0,
-                                                     Counters *cnt0, const uint32_t *__restrict__ off0) {
-    const uint32_t total = offtile_total(pre, ndom, cnt0);
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
-        const int d = offtile_domain(pre, ndom, j);
-        const DomainDesc dd = D[d];
-        DevParams P = P0;
-        P.gx = dd.gx;
-        P.gy = dd.gy;
-        const Lines L{L0.base + dd.pt_off, L0.pitch};
-        float *gm = G0.m + dd.node_base, *gpx = G0.px + dd.node_base, *gpy = G0.py + dd.node_base;
-        const uint32_t i = off0[dd.off_base + (j - (pre ? pre[d] : 0u))];
-        const uint32_t u = L.u(ICE_IDX_UTILITY, i);
-        if (u & kDisabled) continue;
-        uint32_t err = 0;
-        P2GPoint q;
-        load_p2g_point(L, i, L.u(ICE_IDX_CELL, i), P, q);
-        if (!stencil_inside(q.cx, q.cy, P)) {
-            atomicOr(&cnt0[d].err, ICE_ERR_POINT_LEFT_AREA);
-            continue;
-        }
-        float wx[3], wy[3];
-        weights(q.x, wx);
-        weights(q.y, wy);
-        for (int a = -1; a <= 1; a++)
-            for (int b = -1; b <= 1; b++) {
-                float dm, dpx, dpy;
-                p2g_contrib(q, a, b, wx[a + 1] * wy[b + 1], dm, dpx, dpy);
-                const size_t node = (size_t)(q.cy + b) + (size_t)(q.cx + a) * P.gy;
-                unsafeAtomicAdd(&gm[node], dm);
-                unsafeAtomicAdd(&gpx[node], dpx);
-                unsafeAtomicAdd(&gpy[node], dpy);
-                if (__builtin_isnan(dpx) || __builtin_isnan(dpy)) err |= ICE_ERR_GRID_P2G_NAN_VEL;
-                if (__builtin_isnan(dm)) err |= ICE_ERR_GRID_P2G_NAN_MASS;
-            }
-        if (err) atomicOr(&cnt0[d].err, err);   // rare: per item, to the item's domain
-    }
 }
 
 }  // namespace ice
