@@ -241,7 +241,8 @@ def main() -> int:
     elif ms_g2p >= ms_p2g:
         dominant, ms_dom, kname = "g2p", ms_g2p, "k_g2p_tiles" if not args.naive else "k_g2p_naive"
     else:
-        dominant, ms_dom, kname = "p2g", ms_p2g, "k_p2g_tiles" if not args.naive else "k_p2g_naive"
+        # the unfused P2G is the tile pass's P2G-only variant into the frames
+        dominant, ms_dom, kname = "p2g", ms_p2g, "k_tile_pipeline<P2G>" if not args.naive else "k_p2g_naive"
     achieved = B[dominant] / (ms_dom * 1e-3) / 1e9
     value = n_active * args.steps / wall
     # the dominant kernel's bytes per wall-clock substep (every kernel and gap
