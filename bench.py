@@ -140,7 +140,8 @@ def main() -> int:
                          "compressed: the plain batched substep of the surrogate workload's domains (fast flow)")
     ap.add_argument("--surrogate-tol", type=float, default=1e-4, help="yield-violation tolerance of the fallback")
     ap.add_argument("--single-handle", action="store_true",
-                    help="surrogate workload: the surrogate leg only, one handle, eager launches (for profiling)")
+                    help="one handle only (for profiling): the surrogate workload's surrogate leg with eager "
+                         "launches; the batch64 workload without its one-handle-per-domain sequential leg")
     ap.add_argument("--copy-only", type=float, default=0.0,
                     help="only time a plain copy of this many GB (HBM reference / counter calibration)")
     ap.add_argument("--no-secondary", action="store_true",
@@ -321,11 +322,14 @@ def _timed_run(sim, steps: int, graph: bool = True) -> float:
     return time.perf_counter() - t0
 
 
-def measure_batch(steps: int, autotune: bool = True, persistent: bool = False, resort_drift=None) -> dict:
+def measure_batch(steps: int, autotune: bool = True, persistent: bool = False, resort_drift=None,
+                  single_handle: bool = False) -> dict:
     """batch64_uneven: 64 independent domains (sides 128-1024 cells, 25-95 % ice)
     run (a) as one batched handle (one launch per phase, HIP-graph replays) and
     (b) one domain after another with the single-domain path, the reference's
-    one-partition-at-a-time shape, each over the same `steps` substeps."""
+    one-partition-at-a-time shape, each over the same `steps` substeps.
+    single_handle skips (b), which creates one handle per domain (profiled
+    runs use one handle)."""
     from icempm_2d_sp_amd import scenes
     from icempm_2d_sp_amd.gpu import BatchSimulation, Simulation
     t0 = time.time()
@@ -345,24 +349,27 @@ def measure_batch(steps: int, autotune: bool = True, persistent: bool = False, r
     b.close()
     print(f"[bench] batched: {steps} substeps in {wall_b:.3f}s (tile {tile})", file=sys.stderr, flush=True)
     wall_s = 0.0
-    for sc in doms:
-        s1 = Simulation(sc)
-        wall_s += _timed_run(s1, steps)
-        s1.close()
-    print(f"[bench] sequential: {wall_s:.3f}s", file=sys.stderr, flush=True)
+    if not single_handle:
+        for sc in doms:
+            s1 = Simulation(sc)
+            wall_s += _timed_run(s1, steps)
+            s1.close()
+        print(f"[bench] sequential: {wall_s:.3f}s", file=sys.stderr, flush=True)
     return {
         "workload": "batch64_uneven: 64 domains, sides log-uniform 128-1024 cells, 25-95% ice, 4 points/cell, "
                     "per-domain wind 0.05-0.3 Pa, plain FP32 substep",
         "points": n_active, "steps": steps, "tile": f"{tile[0]}x{tile[1]}", "sorts_in_timed_region": sorts_b,
         "value": n_active * steps / wall_b, "unit": "particle-updates/s", "ms_per_step": wall_b * 1000 / steps,
-        "sequential_single_domain": {"value": n_active * steps / wall_s, "ms_per_step": wall_s * 1000 / steps},
-        "batched_over_sequential": wall_s / wall_b,
+        "sequential_single_domain": None if single_handle else {"value": n_active * steps / wall_s,
+                                                                 "ms_per_step": wall_s * 1000 / steps},
+        "batched_over_sequential": None if single_handle else wall_s / wall_b,
     }
 
 
 def bench_batch(args) -> int:
     """batch64_uneven (measure_batch), tile autotuned; value = the batched rate."""
-    r = measure_batch(args.steps, autotune=True, persistent=args.persistent, resort_drift=args.resort_drift)
+    r = measure_batch(args.steps, autotune=True, persistent=args.persistent, resort_drift=args.resort_drift,
+                      single_handle=args.single_handle)
     out = {
         "metric": "particle-updates/s per substep, 64 batched uneven domains (plain FP32 substep)",
         "value": r["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": r["steps"], "warmup": 16,
