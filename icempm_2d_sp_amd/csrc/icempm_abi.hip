@@ -490,6 +490,7 @@ bool sort_ranges_ok(const IceHandle *h, int d) {
 }
 
 int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
+    if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
     Domain &D = h->dom[d];
     const uint32_t n = D.n;
     const uint32_t NC = (uint32_t)D.N;
@@ -555,6 +556,7 @@ int device_sort(IceHandle *h, int d, bool drop, uint32_t *perm_dev) {
 // domain, or the naive path, goes through device_sort.
 int device_sort_many(IceHandle *h, const std::vector<int> &ds) {
     if (ds.empty()) return ICE_OK;
+    if (!h->pts) { h->err = "no points allocated"; return ICE_E_STATE; }
     if (ds.size() == 1 || h->naive()) {
         for (int d : ds) {
             int rc = device_sort(h, d, false, nullptr);

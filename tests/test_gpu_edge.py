@@ -1,7 +1,8 @@
 """GPU edge cases: a domain without points (single handle and inside a
 batch), points exactly on cell boundaries (x or y = +-0.5) against the FP64
-oracle, and the capacity limit (2^30 slots) rejected without losing the
-handle's allocation."""
+oracle, the capacity limit (2^30 slots) rejected without losing the
+handle's allocation, and substeps or sorts on a handle whose points were
+never allocated rejected with an error."""
 import numpy as np
 import pytest
 
@@ -93,3 +94,22 @@ def test_capacity_limit_keeps_the_allocation():
     pts, _, err, _ = sim.download()
     assert pts.shape[1] == sc.n and err == 0
     sim.close()
+
+
+def test_calls_before_allocation_report_an_error():
+    """A handle whose points were never allocated: ice_run, ice_substep and
+    ice_sort return ICE_E_STATE with a message, and launch nothing."""
+    import ctypes as C
+    _print("test_calls_before_allocation_report_an_error")
+    sc = scenes.dynamic_scene(seed=4)
+    lib = abi.load()
+    p = sc.params.to_c()
+    h = C.c_void_p()
+    assert lib.ice_create(C.byref(p), 0, C.byref(h)) == 0 and h
+    try:
+        assert lib.ice_run(h, 0, 4, 1) == -3 and b"no points allocated" in lib.ice_last_error(h)
+        assert lib.ice_substep(h, 0.0, 1) == -3
+        assert lib.ice_sort(h, None, None) == -3
+        assert lib.ice_synchronize(h) == 0
+    finally:
+        lib.ice_destroy(h)
