@@ -320,7 +320,11 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
     __shared__ uint32_t s_rb[RC];
     __shared__ uint32_t s_pre[RC + 1];
     __shared__ int s_next;
-    __shared__ uint32_t s_tally[3];   // the tile's drift, accepted and fallback counts
+    __shared__ uint32_t s_tally[4];   // the tile's drift, accepted, fallback and recorded-state counts
+    // recording: each wave's kept states and the workgroup's first record
+    // slot, double-buffered over the passes of a batch
+    __shared__ uint32_t s_rec[SURR ? 2 : 1][kTileThreads / 64];
+    __shared__ unsigned long long s_rec_base[SURR ? 2 : 1];
 
     // Workgroup b takes tile-table entry xcd_remap(b).  PERSIST: the launch is
     // persistent, workgroup b (on XCD b % 8) takes the next entry of that
@@ -406,11 +410,11 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
             }
         }
         for (int k = threadIdx.x; k < 3 * Fr::NN; k += kTileThreads) s_acc[k] = 0.f;
-        if (threadIdx.x < 3) s_tally[threadIdx.x] = 0u;
+        if (threadIdx.x < 4) s_tally[threadIdx.x] = 0u;
         // wave-uniform counts, added to the domain's counters once per tile:
         // one global atomic per wave on a domain-wide address serialises at
         // the L2 when most waves count (fast flow, the learned projection)
-        uint32_t w_drift = 0, w_acc = 0, w_fb = 0;
+        uint32_t w_drift = 0, w_acc = 0, w_fb = 0, w_seen = 0;
 
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         for (int rd = 0; rd < NROUND; rd++) {
@@ -559,12 +563,21 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                                                                      (uint64_t)__double_as_longlong(time));
                                 const bool keep = want && hsh < SA.rec_keep;
                                 const uint64_t bw = __ballot(want), bk = __ballot(keep);
-                                unsigned long long base = 0;
-                                if (lane == 0 && bw) {
-                                    atomicAdd(&SA.rec_count[0], (unsigned long long)__popcll(bw));
-                                    if (bk) base = atomicAdd(&SA.rec_count[1], (unsigned long long)__popcll(bk));
+                                w_seen += (uint32_t)__popcll(bw);
+                                // the workgroup's kept states take one block of record
+                                // slots (one atomic per workgroup and pass, not per wave)
+                                const int rb = p & 1;
+                                if (lane == 0) s_rec[rb][wave] = (uint32_t)__popcll(bk);
+                                __syncthreads();
+                                if (threadIdx.x == 0) {
+                                    uint32_t tot = 0;
+#pragma unroll
+                                    for (int w = 0; w < kTileThreads / 64; w++) tot += s_rec[rb][w];
+                                    s_rec_base[rb] = tot ? atomicAdd(&SA.rec_count[1], (unsigned long long)tot) : 0ull;
                                 }
-                                base = __shfl(base, 0, 64);
+                                __syncthreads();
+                                unsigned long long base = s_rec_base[rb];
+                                for (int w = 0; w < wave; w++) base += s_rec[rb][w];
                                 const uint64_t slot_rec = base + __popcll(bk & ((1ull << lane) - 1ull));
                                 if (keep && slot_rec < SA.rec_cap)
                                     for (int k = 0; k < kMlpIn; k++) SA.rec_x[slot_rec * kMlpIn + k] = xr[k];
@@ -719,12 +732,14 @@ __global__ __launch_bounds__(kTileThreads, (tile_min_blocks<TW, TH, SURR>())) vo
                 if (w_drift) atomicAdd(&s_tally[0], w_drift);
                 if (w_acc) atomicAdd(&s_tally[1], w_acc);
                 if (w_fb) atomicAdd(&s_tally[2], w_fb);
+                if (w_seen) atomicAdd(&s_tally[3], w_seen);
             }
             __syncthreads();
             if (threadIdx.x == 0) {
                 if (s_tally[0]) atomicAdd(&cnt->n_drift, s_tally[0]);
                 if (s_tally[1]) atomicAdd(&cnt->pad[0], s_tally[1]);
                 if (s_tally[2]) atomicAdd(&cnt->pad[1], s_tally[2]);
+                if (s_tally[3]) atomicAdd(&SA.rec_count[0], (unsigned long long)s_tally[3]);
             }
         }
         // the frame, every node once (coalesced).  A NaN P2G contribution
