@@ -2069,6 +2069,9 @@ int ice_mlp_record(IceHandle *h, uint32_t capacity, float keep_fraction) {
     HIPCHK(h, hipMalloc(&h->mlp_rec_cnt, 2 * sizeof(unsigned long long)));
     int rc = launch_fill_u32(h, reinterpret_cast<uint32_t *>(h->mlp_rec_cnt), 4, 0u);
     if (rc) return rc;
+    // rows start as NaN: a slot counted as stored but never written shows
+    if ((rc = launch_fill_u32(h, reinterpret_cast<uint32_t *>(h->mlp_rec), kMlpIn * (size_t)capacity, 0x7fc00000u)))
+        return rc;
     h->mlp_rec_cap = capacity;
     h->mlp_rec_keep = (uint32_t)std::min(4294967295.0, std::floor((double)keep_fraction * 4294967296.0));
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -246,3 +246,38 @@ def test_batch_deploy_toggle_matches_exact_batch():
         _print("domain", d, "Fe max", rep["Fe"]["max"], "bad", rep["Fe"]["bad"])
     t.close()
     ex.close()
+
+
+def test_recording_counts_and_rows():
+    """The recording of ridge-capped plastic trial states on a small compressed
+    batch: every such state is counted (a tally per tile) and every kept one
+    takes one record slot (a block of slots per workgroup and pass).  With
+    everything kept the two counts agree and every stored row is a written,
+    finite feature vector (rows start as NaN); a keep fraction of 1/4 keeps
+    that share of the states within 6 binomial sigma; a capacity below the
+    state count stores exactly the capacity."""
+    from icempm_2d_sp_amd.gpu import BatchSimulation
+    print("test_recording_counts_and_rows", flush=True)
+    doms = scenes.batch64_compressed(n_domains=4, max_side=256)
+    b = BatchSimulation(doms)
+    b.run(0, 16, graph=True)                    # the compression develops
+    b.mlp_record(1 << 22, 1.0)
+    b.run(16, 4, graph=False)
+    seen, stored = b.mlp_record_count()
+    x = b.mlp_records(stored)
+    print("keep 1:", seen, stored, flush=True)
+    assert seen > 10_000 and stored == seen and x.shape[0] == stored and np.isfinite(x).all()
+    b.mlp_record(1 << 22, 0.25)
+    b.run(20, 4, graph=False)
+    seen, stored = b.mlp_record_count()
+    x = b.mlp_records(stored)
+    sigma = (0.25 * 0.75 / seen) ** 0.5
+    print("keep 1/4:", seen, stored, stored / seen, flush=True)
+    assert abs(stored / seen - 0.25) <= 6 * sigma and np.isfinite(x).all()
+    b.mlp_record(1000, 1.0)
+    b.run(24, 2, graph=False)
+    seen, stored = b.mlp_record_count()
+    x = b.mlp_records(2000)
+    assert seen > 1000 and stored == 1000 and x.shape[0] == 1000 and np.isfinite(x).all()
+    b.mlp_record(0, 0.0)
+    b.close()
