@@ -2266,8 +2266,9 @@ int ice_mlp_train(IceHandle *h, uint32_t steps, uint32_t batch, float lr, uint64
     if (steps && (rc = launch_fill(h, h->mlp_loss, steps))) return rc;
     const MlpDev M{h->mlp_pk, h->mlp_p, h->mlp_ipk};
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    // the gradient starts at zero; each Adam step clears what it consumed
+    if (steps && (rc = launch_fill(h, h->mlp_g, kMlpParams))) return rc;
     for (uint32_t k = 0; k < steps; k++) {
-        if ((rc = launch_fill(h, h->mlp_g, kMlpParams))) return rc;
         hipLaunchKernelGGL(k_mlp_train, dim3(blocks_for(batch, 64)), dim3(64), 0, h->stream, M, h->mlp_norm,
                            (const float *)h->mlp_x, (const float *)h->mlp_t, h->mlp_n, batch, seed, k, 0, h->mlp_g,
                            h->mlp_loss + k);
@@ -2276,7 +2277,7 @@ int ice_mlp_train(IceHandle *h, uint32_t steps, uint32_t batch, float lr, uint64
         const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)h->mlp_adam_t)));
         const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)h->mlp_adam_t)));
         hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kMlpParams, 256)), dim3(256), 0, h->stream, h->mlp_p,
-                           (const float *)h->mlp_g, h->mlp_m, h->mlp_v, lr, b1, b2, eps, c1, c2);
+                           h->mlp_g, h->mlp_m, h->mlp_v, lr, b1, b2, eps, c1, c2);
         LAUNCHCHK(h, "k_mlp_adam");
         if ((rc = mlp_pack(h, k + 1 == steps))) return rc;
     }

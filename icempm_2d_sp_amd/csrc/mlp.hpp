@@ -582,13 +582,15 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDev M, MlpNorm N, const flo
     }
 }
 
-// Adam (bias-corrected) on the FP32 master parameters.
-__global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ p, const float *__restrict__ g,
+// Adam (bias-corrected) on the FP32 master parameters; clears the gradient
+// it consumed for the next step's accumulation.
+__global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ p, float *__restrict__ g,
                                                   float *__restrict__ m, float *__restrict__ v, float lr, float b1,
                                                   float b2, float eps, float c1, float c2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kMlpParams) return;
     const float gi = g[i];
+    g[i] = 0.f;
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
