@@ -128,6 +128,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--naive", action="store_true", help="reference launch structure (unsorted, atomics)")
     ap.add_argument("--unfused", action="store_true", help="separate P2G and update + G2P tile kernels")
+    ap.add_argument("--det", action="store_true",
+                    help="deterministic mode (fixed-point P2G, bit-identical results; unfused kernels)")
     ap.add_argument("--persistent", action="store_true", help="tile passes as persistent launches (work queues)")
     ap.add_argument("--resort-drift", type=int, default=None,
                     help="re-sort a domain when this %% of its points left their sorted cell (0 = off; library default 30)")
@@ -188,8 +190,8 @@ def main() -> int:
     print(f"[bench] scene {gx}x{gy}: {scene.n} points, generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
 
     tw, th = (int(v) for v in args.tile.split("x"))
-    flags = scene.params.flags | (abi.FLAG_NAIVE if args.naive else 0)
-    pipelined = not (args.naive or args.unfused)
+    flags = scene.params.flags | (abi.FLAG_NAIVE if args.naive else 0) | (abi.FLAG_DETERMINISTIC if args.det else 0)
+    pipelined = not (args.naive or args.unfused or args.det)
     opts = {"pipelined": int(pipelined), "persistent": int(args.persistent)}
     if args.resort_drift is not None:
         opts["resort_drift"] = args.resort_drift
@@ -243,7 +245,8 @@ def main() -> int:
         dominant, ms_dom, kname = "g2p", ms_g2p, "k_g2p_tiles" if not args.naive else "k_g2p_naive"
     else:
         # the unfused P2G is the tile pass's P2G-only variant into the frames
-        dominant, ms_dom, kname = "p2g", ms_p2g, "k_tile_pipeline<P2G>" if not args.naive else "k_p2g_naive"
+        dominant, ms_dom, kname = "p2g", ms_p2g, ("k_p2g_naive" if args.naive else
+                                                  "k_p2g_tiles_det" if args.det else "k_tile_pipeline<P2G>")
     achieved = B[dominant] / (ms_dom * 1e-3) / 1e9
     value = n_active * args.steps / wall
     # the dominant kernel's bytes per wall-clock substep (every kernel and gap
@@ -288,8 +291,10 @@ def main() -> int:
                          "random polygonal ice field (~95%), GRF wind stress (mean 0.2 Pa, std 0.05 Pa, "
                          "corr 200 cells), lognormal cohesion (median 1e5 Pa, sigma 0.5), dt=1e-4 s"),
             "points": n_active, "nodes": nodes, "tile": args.tile,
-            "path": "naive" if args.naive else ("sorted-unfused" if args.unfused else
-                                                ("sorted-pipelined-persistent" if args.persistent else "sorted-pipelined")),
+            "path": "naive" if args.naive else ("sorted-deterministic" if args.det else
+                                                ("sorted-unfused" if args.unfused else
+                                                 ("sorted-pipelined-persistent" if args.persistent else
+                                                  "sorted-pipelined"))),
             "sorts_in_timed_region": sorts_timed,
             "error_word": err,
         },
